@@ -255,6 +255,29 @@ int omc_read_3ddose(const char *path, int *isize, int *jsize, int *ksize,
 int64_t omc_sparse_column(const double *dose, int64_t nvox, double rel_thr,
                           int64_t *rows, double *vals, int64_t cap);
 
+/* ---- omc_dosxyz problem from an .inp file ---------------------------------------- */
+typedef struct {
+    omc_inp           inp;
+    omc_phantom       ph;
+    omc_tables        tab;
+    omc_regions       reg;
+    omc_source_dosxyz src;
+    int  nsplit;
+    int  seeds[2];                 /* "rng seeds" */
+    int  nhist, nbatch, nperbatch; /* nhist = nperbatch * nbatch */
+    char output_folder[OMC_INP_BUF];
+} omc_dosxyz_problem;
+
+/* Reads <stem>.inp and performs initPhantom, initMediaData, initSource,
+ * initRegions and initVrt (omc_dosxyz.c:1155-1231). */
+int  omc_dosxyz_problem_load(omc_dosxyz_problem *p, const char *inp_stem);
+/* Same, with p->inp already filled (e.g. by omc_inp_parse_text). */
+int  omc_dosxyz_problem_from_inp(omc_dosxyz_problem *p);
+void omc_dosxyz_problem_free(omc_dosxyz_problem *p);
+/* Pieces shared with the matRad driver. */
+int  omc_problem_media(omc_dosxyz_problem *p, const omc_inp *inp);
+int  omc_problem_spectrum(omc_spectrum *s, const omc_inp *inp);
+
 #ifdef __cplusplus
 }
 #endif

@@ -176,7 +176,7 @@ def write_spectrum(path, edges=None, weights=None, title="synthetic 6 MV"):
         f.write(title + "\n")
         f.write(f"   {len(edges)}  0.0  0\n")
         for e, w in zip(edges, weights):
-            f.write(f"  {e!r}  {w!r}\n")
+            f.write(f"  {float(e)!r}  {float(w)!r}\n")
     return Path(path)
 
 
