@@ -1,0 +1,155 @@
+/*
+ * omc_dev.h - C ABI of libomc_hip.so, the MI355X (gfx950) transport engine.
+ *
+ * This is the drop-in boundary for ompMC's shower() path.  In ompMC the user
+ * code drives transport one history at a time:
+ *
+ *     for each batch:                                   omc_dosxyz.c:1237-1263
+ *         #pragma omp parallel for                      omc_matrad.c:1389-1414
+ *         for each history: initHistory(); shower();    ompmc.c:5436-5447
+ *         accumEndep();                                 omc_dosxyz.c:696-717
+ *
+ * with shower() calling back into howfar/hownear/ausgab (ompmc.h:37-39).
+ * A GPU cannot call host callbacks, so the boundary moves up one level: the
+ * engine takes the whole batch loop -- source sampling, transport in egsphant
+ * voxel geometry, scoring -- and returns the batch sums accumEndep would have
+ * accumulated.  Host code (omc_host.h) keeps building tables, phantoms and
+ * sources exactly as ompMC does, and keeps the statistics and output files.
+ *
+ * Conventions: every call returns OMC_OK (0) or a negative OMC_E* code from
+ * omc_host.h and never exits; omc_dev_last_error() describes the failure.
+ * Inputs are copied (the caller keeps ownership); outputs go to caller
+ * allocated host arrays.  One host thread per context.
+ */
+#ifndef OMC_DEV_H
+#define OMC_DEV_H
+
+#include "omc_host.h"
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct omc_dev_ctx omc_dev_ctx;
+
+enum { OMC_PREC_FP64 = 0,   /* all transport in FP64 (plain GPU kernel) */
+       OMC_PREC_FP32 = 1 }; /* FP32 transport, FP64 tallies and CSDA inversion */
+
+typedef struct {
+    int     device;          /* HIP device ordinal */
+    int     precision;       /* OMC_PREC_FP64 / OMC_PREC_FP32 */
+    int64_t hbm_budget;      /* bytes the context may hold; 0 = 90% of free HBM */
+    int64_t chunk;           /* primaries per transport chunk; 0 = automatic */
+} omc_dev_opts;
+
+/* Device counters of one run (all sums over the run). */
+typedef struct {
+    int64_t histories;
+    int64_t photon_steps;       /* photon voxel steps (howfar calls) */
+    int64_t electron_steps;     /* electron ustep iterations */
+    int64_t deposits;           /* ausgab calls */
+    int64_t photons_pushed, electrons_pushed;
+    int64_t generations;
+    int64_t chunks;
+    double  transport_ms;       /* device time of the transport kernels */
+    double  photon_kernel_ms, electron_kernel_ms, source_kernel_ms;
+    int64_t photon_launches, electron_launches;
+    int64_t killed;             /* particles stopped by the step-limit watchdog */
+} omc_dev_stats;
+
+/* Context: device, stream, HBM budget, precision.  Replaces the per-thread
+ * initStack/initRandom set-up (ompmc.c:47-64, omc_random.c:58-119). */
+int  omc_dev_create(omc_dev_ctx **ctx, const omc_dev_opts *opts);
+int  omc_dev_destroy(omc_dev_ctx *ctx);
+const char *omc_dev_last_error(void);
+int  omc_dev_precision(const omc_dev_ctx *ctx);
+
+/* Deep copy of every table shower() reads (photon_data, rayleigh_data,
+ * pair_data, electron_data, spin_data, mscat_data, pegs_data;
+ * ompmc.h:104-401) into the device layout.  Replaces the global tables that
+ * initMediaData fills (ompmc.c:5450-5487). */
+int  omc_dev_upload_media(omc_dev_ctx *ctx, const omc_tables *t);
+
+/* Resident voxel phantom and its region data (omc_dosxyz.c:62-175,
+ * 890-962).  Several phantoms may be resident; each gets an id. */
+int  omc_dev_upload_phantom(omc_dev_ctx *ctx, const omc_phantom *ph,
+                            const omc_regions *reg, int *phantom_id);
+int  omc_dev_release_phantom(omc_dev_ctx *ctx, int phantom_id);
+
+typedef struct {
+    int      phantom_id;
+    int      nsplit;            /* photon splitting; the device path requires 1 */
+    uint32_t seed[2];           /* run seed (e.g. the .inp "rng seeds") */
+    int64_t  hist0;             /* index of the first history (stream id) */
+    int64_t  nperbatch;         /* histories per statistical batch */
+    int      nbatch;            /* number of batches */
+    double  *ledger;            /* optional [3*nperbatch*nbatch]: per history
+                                   energy deposited in the phantom, deposited
+                                   in region 0, carried out by photons */
+    omc_dev_stats *stats;       /* optional */
+} omc_dev_run_params;
+
+/* The omc_dosxyz batch loop on the device: nbatch x nperbatch histories of
+ * initHistory (omc_dosxyz.c:964-1068) + shower().  On return sum[i] and
+ * sum2[i] ([nreg] host arrays, region 0 included) have been increased by
+ * the per-batch energy and its square, exactly what accumEndep adds
+ * (omc_dosxyz.c:696-717), so omc_batch_results() gives the .3ddose values. */
+int  omc_dev_run_dosxyz(omc_dev_ctx *ctx, const omc_source_dosxyz *src,
+                        const omc_dev_run_params *rp, double *sum, double *sum2);
+
+/* matRad dose influence: every beamlet of `src` in one device pass
+ * (omc_matrad.c:1389-1493).  nhist[b] histories for beamlet b split into
+ * rp->nbatch batches.  Outputs per beamlet b (caller arrays):
+ *   sum [b*nreg + i], sum2[b*nreg + i]   batch sums (may be NULL if only
+ *                                        the sparse output is wanted)
+ * The sparse thresholded Dij is produced by omc_dev_beamlets_sparse(). */
+int  omc_dev_run_beamlets(omc_dev_ctx *ctx, const omc_source_beamlets *src,
+                          const int64_t *nhist, const omc_dev_run_params *rp,
+                          double *sum, double *sum2);
+
+/* Same beamlet pass, with the matRad Dij assembled on the device: for each
+ * beamlet the mean dose per history (omc_matrad.c:1418 normalisation by
+ * the beamlet's history count) above rel_thr * beamlet max is emitted as one
+ * CSC column.  jcs has nbeamlets+1 entries; irs/vals have `cap` entries;
+ * *nnz receives the number written (OMC_ERANGE if cap is too small). */
+int  omc_dev_beamlets_sparse(omc_dev_ctx *ctx, const omc_source_beamlets *src,
+                             const int64_t *nhist, const omc_dev_run_params *rp,
+                             double rel_thr, int64_t *jcs, int64_t *irs,
+                             double *vals, int64_t cap, int64_t *nnz);
+
+/* ---- serving: continuous batching of dose requests ----------------------- */
+typedef struct {
+    int      phantom_id;
+    int      kind;              /* 0 = dosxyz field (src_dosxyz), 1 = beamlet */
+    const omc_source_dosxyz   *src_dosxyz;
+    const omc_source_beamlets *src_beamlets;
+    int      beamlet;           /* for kind 1 */
+    int64_t  nhist;             /* histories (split into nbatch groups) */
+    int      nbatch;
+    uint32_t seed[2];
+} omc_dev_request;
+
+typedef struct {
+    int     ticket;
+    int     done;               /* 1 when finished */
+    double  submit_ms, admit_ms, finish_ms;   /* engine clock */
+    int64_t nhist;
+} omc_dev_result;
+
+/* Queues a request; it is admitted once its dose grid and particle share fit
+ * the HBM budget.  Returns a ticket >= 0. */
+int  omc_dev_submit(omc_dev_ctx *ctx, const omc_dev_request *req);
+/* Advances the engine by one transport generation (admits waiting requests
+ * whose memory fits, launches source + photon + electron kernels for all
+ * in-flight requests).  Returns the number of requests still unfinished. */
+int  omc_dev_step(omc_dev_ctx *ctx);
+/* Result of a ticket; when done, sum/sum2 ([nreg] each, may be NULL) get the
+ * request's batch sums and the request's device memory is released. */
+int  omc_dev_poll(omc_dev_ctx *ctx, int ticket, omc_dev_result *res,
+                  double *sum, double *sum2);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* OMC_DEV_H */

@@ -1,0 +1,242 @@
+// Device-side building blocks for the gfx950 transport engine: the
+// per-particle Philox4x32-10 stream, 64-wide wavefront helpers, and the
+// layouts of tables, phantoms, jobs and particle queues in HBM.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "omc_host.h"
+
+namespace omc {
+
+constexpr double RM = OMC_RM;
+constexpr int MXGE = OMC_MXGE;
+constexpr int MXEKE = OMC_MXEKE;
+constexpr int MAXMED = OMC_MXMED;
+
+// ---------------------------------------------------------------- RNG -----
+// Philox4x32-10 (Salmon et al., SC'11).  Each particle owns a stream: key
+// (k0,k1) and a draw counter `ctr`; draw d uses word d&3 of block d>>2 and
+// becomes the 24-bit uniform (w >> 8) * 2^-24, the same granularity as the
+// RANMAR numbers ompMC uses.  A secondary's key is the Philox block
+// (parent ctr, 1, 0, 0) under the parent key, taken after the interaction's
+// last draw, so every history is independent of processing order.
+__device__ __forceinline__ void philox10(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                                         uint32_t k0, uint32_t k1, uint32_t out[4])
+{
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        if (r) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
+        uint32_t lo0 = 0xD2511F53u * c0, hi0 = __umulhi(0xD2511F53u, c0);
+        uint32_t lo1 = 0xCD9E8D57u * c2, hi1 = __umulhi(0xCD9E8D57u, c2);
+        uint32_t n0 = hi1 ^ c1 ^ k0;
+        uint32_t n2 = hi0 ^ c3 ^ k1;
+        c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+struct Rng {
+    uint32_t k0, k1, ctr;
+    uint32_t blk;            // block held in w[] (0xffffffff = none)
+    uint32_t w0, w1, w2, w3;
+
+    __device__ __forceinline__ void reset_cache() { blk = 0xffffffffu; }
+
+    __device__ __forceinline__ uint32_t next_u24()
+    {
+        uint32_t b = ctr >> 2;
+        if (b != blk) {
+            uint32_t o[4];
+            philox10(b, 0u, 0u, 0u, k0, k1, o);
+            w0 = o[0]; w1 = o[1]; w2 = o[2]; w3 = o[3];
+            blk = b;
+        }
+        uint32_t s = ctr & 3u;
+        uint32_t w = s == 0 ? w0 : s == 1 ? w1 : s == 2 ? w2 : w3;
+        ctr++;
+        return w >> 8;
+    }
+    __device__ __forceinline__ void child(uint32_t &c0, uint32_t &c1) const
+    {
+        uint32_t o[4];
+        philox10(ctr, 1u, 0u, 0u, k0, k1, o);
+        c0 = o[0]; c1 = o[1];
+    }
+};
+
+// next uniform of a particle stream, in [0,1) with 24-bit granularity
+template <typename R> __device__ __forceinline__ R urand(Rng &r)
+{
+    return (R)r.next_u24() * (R)(1.0 / 16777216.0);
+}
+
+__device__ __forceinline__ void primary_key(uint32_t s0, uint32_t s1, uint64_t hid, uint32_t &k0, uint32_t &k1)
+{
+    uint32_t o[4];
+    philox10((uint32_t)hid, (uint32_t)(hid >> 32), 2u, 0u, s0, s1, o);
+    k0 = o[0]; k1 = o[1];
+}
+
+// ------------------------------------------------------------ wavefront ----
+__device__ __forceinline__ int lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ uint32_t lanes_below(uint64_t m)
+{
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+// One atomic per wavefront for all active lanes that call it together; every
+// calling lane gets a distinct index.  Works in divergent code (only the
+// currently active lanes take part).
+__device__ __forceinline__ uint32_t wave_reserve(uint32_t *counter, uint32_t n_per_lane)
+{
+    uint64_t m = __ballot(1);
+    int leader = __ffsll((unsigned long long)m) - 1;
+    uint32_t total = (uint32_t)__popcll(m) * n_per_lane;
+    uint32_t base = 0;
+    if (lane_id() == leader) base = atomicAdd(counter, total);
+    base = __shfl(base, leader);
+    return base + lanes_below(m) * n_per_lane;
+}
+
+// ------------------------------------------------------------- layouts ----
+template <typename R> struct R2 { R c1, c0; };           // pwlf slope / intercept
+template <typename R> struct R4 { R a1, a0, b1, b0; };   // two pwlf pairs
+
+// Per (medium, charge, bin) electron step data used at every CH step
+// (sig, tmxs, eta, blcce) and by msdist / the interaction choice.
+template <typename R> struct ElecBin {
+    R sig1, sig0, tmxs1, tmxs0;
+    R eta1, eta0, blc1, blc0;
+    R q1c1, q1c0, q2c1, q2c0;
+    R br11, br10, br21, br20;
+};
+
+// CSDA data, FP64 in every precision mode: restricted stopping power,
+// range to the first table energy, and that energy.
+struct ElecCsda {
+    double dedx1, dedx0, range, earr;
+};
+
+// multiple-scattering surface entry: alias threshold, alias index, u node
+struct MsEntry {
+    double ums, wms;
+    int ims, pad;
+};
+
+// Per-medium scalars and the small tables.
+struct MediumScalars {
+    double ge0, ge1, eke0, eke1, blcc, xcc, esig_e, psig_e;
+    double te, thmoll, ap, delcm, zbrang, bpar0, bpar1, pad;
+    int mono_e, mono_p, meke, pad2;
+    double dl1[8], dl2[8], dl3[8], dl4[8], dl5[8], dl6[8];
+};
+
+struct SmallTables {
+    int nmed;
+    int pad;
+    double b2spin_min, dbeta2i, espml, dleneri, dqq1i, dllambi, dqmsi;
+    // Rayleigh: ompMC samples with medium-0 form factors (ompmc.c:1120-1131)
+    double xgrid[OMC_MXRAYFF], fcum[OMC_MXRAYFF], b_array[OMC_MXRAYFF], c_array[OMC_MXRAYFF];
+    int i_array[OMC_RAYCDFSIZE];
+    MediumScalars med[MAXMED];
+};
+
+// Device tables for precision R.  Photon bins are interleaved so one voxel
+// step reads one 32-byte record.
+template <typename R> struct DevTables {
+    const SmallTables *small;
+    const R4<R> *ph_mfp;     // {gmfp1, gmfp0, cohe1, cohe0} [imed*MXGE + bin]
+    const R4<R> *ph_br;      // {gbr11, gbr10, gbr21, gbr20}
+    const R2<R> *ph_pmax;    // {pmax1, pmax0}
+    const ElecBin<R> *el;    // [(imed*2 + qel)*MXEKE + bin]
+    const ElecCsda *csda;    // [(imed*2 + qel)*MXEKE + bin]
+    const R *spin;           // [imed*32768 + qel*16384 + i*512 + j*32 + k]
+    const MsEntry *ms;       // [i*256 + j*32 + k]
+};
+
+// Resident phantom: region r = voxel + 1, region 0 = outside.
+template <typename R> struct DevPhantom {
+    int isize, jsize, ksize, nreg;
+    const double *xb, *yb, *zb;
+    const uint8_t *med;      // [nreg] medium index, 0xff = vacuum / outside
+    const R *rhof;           // [nreg]
+    double ecut[MAXMED + 1]; // [imed + 1]; entry 0 is vacuum / outside
+    double pcut[MAXMED + 1];
+};
+
+// One source job (a dosxyz field, one beamlet, one serving request).
+struct DevJob {
+    int phantom;             // index into the phantom table
+    int kind;                // 0 dosxyz field, 1 beamlet
+    int charge;
+    int spectrum;
+    double energy;           // mono energy
+    const double *cdf1, *cdf2;   // [1000] inverse CDF (device)
+    double deltak;
+    // dosxyz field
+    double ssd, xinl, yinl, xsize, ysize;
+    int ixinl, iyinl;
+    // beamlet
+    double src[3], corner[3], side1[3], side2[3];
+    // histories
+    uint64_t hist0;          // stream index of the first history
+    int64_t nperbatch;
+    uint32_t s0, s1;         // seed
+    double *tally;           // [slot*nreg + region], slot = batch
+    double *ledger;          // optional [3*nhist]
+};
+
+// Structure-of-arrays particle queue.
+template <typename R> struct Queue {
+    R *e, *x, *y, *z, *u, *v, *w, *wt;
+    int32_t *ir;
+    uint32_t *meta;          // job << 2 | (iq + 1)
+    uint32_t *hist;          // history index within the job
+    uint32_t *k0, *k1, *ctr;
+    uint32_t *count;         // device tail counter
+    uint32_t cap;
+};
+
+template <typename R> struct Particle {
+    R e, x, y, z, u, v, w, wt;
+    int ir, iq;
+    uint32_t job, hist;
+    Rng rng;
+};
+
+template <typename R>
+__device__ __forceinline__ void q_load(const Queue<R> &q, uint32_t i, Particle<R> &p)
+{
+    p.e = q.e[i]; p.x = q.x[i]; p.y = q.y[i]; p.z = q.z[i];
+    p.u = q.u[i]; p.v = q.v[i]; p.w = q.w[i]; p.wt = q.wt[i];
+    p.ir = q.ir[i];
+    uint32_t m = q.meta[i];
+    p.job = m >> 2;
+    p.iq = (int)(m & 3u) - 1;
+    p.hist = q.hist[i];
+    p.rng.k0 = q.k0[i]; p.rng.k1 = q.k1[i]; p.rng.ctr = q.ctr[i];
+    p.rng.reset_cache();
+}
+
+template <typename R>
+__device__ __forceinline__ void q_store(const Queue<R> &q, uint32_t i, const Particle<R> &p)
+{
+    q.e[i] = p.e; q.x[i] = p.x; q.y[i] = p.y; q.z[i] = p.z;
+    q.u[i] = p.u; q.v[i] = p.v; q.w[i] = p.w; q.wt[i] = p.wt;
+    q.ir[i] = p.ir;
+    q.meta[i] = (p.job << 2) | (uint32_t)(p.iq + 1);
+    q.hist[i] = p.hist;
+    q.k0[i] = p.rng.k0; q.k1[i] = p.rng.k1; q.ctr[i] = p.rng.ctr;
+}
+
+// Counters and error flags shared by one launch.
+struct DevStatus {
+    uint32_t overflow;       // a push found its queue full
+    uint32_t killed;         // particles stopped by the per-particle step limit
+    unsigned long long photon_steps, electron_steps, deposits;
+};
+
+}  // namespace omc

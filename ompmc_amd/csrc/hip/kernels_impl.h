@@ -1,0 +1,133 @@
+// Kernel bodies shared by the FP64 (k64.hip) and FP32 (k32.hip) builds:
+// source sampling, batch folding and the launch wrappers the engine calls.
+#pragma once
+
+#include "launch.h"
+#include "transport.h"
+
+namespace omc {
+
+// initHistory for a dosxyz field (omc_dosxyz.c:964-1068) or a matRad beamlet
+// (omc_matrad.c:1084-1254): one thread per history, primary pushed to the
+// photon or charged-particle queue.
+template <typename R>
+__global__ void __launch_bounds__(256) k_source(const DevJob *jobs, const DevPhantom<R> *phs, int job,
+                                                uint64_t first, uint32_t n, Queue<R> qp, Queue<R> qe,
+                                                DevStatus *st)
+{
+    uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const DevJob &J = jobs[job];
+    const DevPhantom<R> &g = phs[J.phantom];
+    const uint64_t h = first + t;
+    Particle<R> p;
+    p.job = (uint32_t)job;
+    p.hist = (uint32_t)h;
+    primary_key(J.s0, J.s1, J.hist0 + h, p.rng.k0, p.rng.k1);
+    p.rng.ctr = 0u;
+    p.rng.reset_cache();
+    p.iq = J.charge;
+    R ein;
+    if (J.spectrum) {
+        R r1 = urand<R>(p.rng), r2 = urand<R>(p.rng);
+        int k = (int)fmin(J.deltak * (double)r1, J.deltak - 1.0);
+        ein = (R)J.cdf1[k] + r2 * (R)J.cdf2[k];
+    } else {
+        ein = (R)J.energy;
+    }
+    p.e = (p.iq != 0) ? ein + (R)RM : ein;
+    p.wt = (R)1.0;
+    if (J.kind == 0) {
+        R rxyz;
+        const R ssd = (R)J.ssd;
+        if (J.xsize == 0.0 || J.ysize == 0.0) {
+            p.x = (R)J.xinl;
+            p.y = (R)J.yinl;
+            rxyz = m_sqrt(sq(ssd) + sq(p.x) + sq(p.y));
+            p.w = ssd / rxyz;
+        } else {
+            R fw, r3;
+            do {
+                r3 = urand<R>(p.rng);
+                p.x = r3 * (R)J.xsize + (R)J.xinl;
+                r3 = urand<R>(p.rng);
+                p.y = r3 * (R)J.ysize + (R)J.yinl;
+                r3 = urand<R>(p.rng);
+                rxyz = m_sqrt(ssd * ssd + p.x * p.x + p.y * p.y);
+                p.w = ssd / rxyz;
+                fw = p.w * p.w * p.w;
+            } while (r3 >= fw);
+        }
+        p.z = (R)g.zb[0];
+        p.u = p.x / rxyz;
+        p.v = p.y / rxyz;
+        int ix, iy;
+        if (J.xsize == 0.0) {
+            ix = J.ixinl;
+        } else {
+            ix = J.ixinl - 1;
+            while (((R)g.xb[ix + 1] < p.x) && ix < g.isize - 1) ix++;
+        }
+        if (J.ysize == 0.0) {
+            iy = J.iyinl;
+        } else {
+            iy = J.iyinl - 1;
+            while (((R)g.yb[iy + 1] < p.y) && iy < g.jsize - 1) iy++;
+        }
+        p.ir = 1 + ix + iy * g.isize;
+    } else {
+        R r1 = urand<R>(p.rng), r2 = urand<R>(p.rng);
+        R xiso = r1 * (R)J.side1[0] + r2 * (R)J.side2[0] + (R)J.corner[0];
+        R yiso = r1 * (R)J.side1[1] + r2 * (R)J.side2[1] + (R)J.corner[1];
+        R ziso = r1 * (R)J.side1[2] + r2 * (R)J.side2[2] + (R)J.corner[2];
+        R sx = (R)J.src[0], sy = (R)J.src[1], sz = (R)J.src[2];
+        R vnorm = m_sqrt(sq(xiso - sx) + sq(yiso - sy) + sq(ziso - sz));
+        R u = -(xiso - sx) / vnorm, v = -(yiso - sy) / vnorm, w = -(ziso - sz) / vnorm;
+        R ustep = (R)1.0E5, dist;
+        if (u > (R)0) { dist = ((R)g.xb[g.isize] - xiso) / u; if (dist < ustep) ustep = dist; }
+        if (u < (R)0) { dist = -(xiso - (R)g.xb[0]) / u; if (dist < ustep) ustep = dist; }
+        if (v > (R)0) { dist = ((R)g.yb[g.jsize] - yiso) / v; if (dist < ustep) ustep = dist; }
+        if (v < (R)0) { dist = -(yiso - (R)g.yb[0]) / v; if (dist < ustep) ustep = dist; }
+        if (w > (R)0) { dist = ((R)g.zb[g.ksize] - ziso) / w; if (dist < ustep) ustep = dist; }
+        if (w < (R)0) { dist = -(ziso - (R)g.zb[0]) / w; if (dist < ustep) ustep = dist; }
+        p.x = xiso + ustep * u;
+        p.y = yiso + ustep * v;
+        p.z = ziso + ustep * w;
+        p.u = -u; p.v = -v; p.w = -w;
+        const R tiny = (R)(2.0 * 2.2250738585072014e-308);
+        if (p.x < (R)g.xb[0]) p.x = (R)g.xb[0] + tiny;
+        if (p.x > (R)g.xb[g.isize]) p.x = (R)g.xb[g.isize] - tiny;
+        if (p.y < (R)g.yb[0]) p.y = (R)g.yb[0] + tiny;
+        if (p.y > (R)g.yb[g.jsize]) p.y = (R)g.yb[g.jsize] - tiny;
+        if (p.z < (R)g.zb[0]) p.z = (R)g.yb[0] + tiny;      // reference's ybounds (omc_matrad.c:1225)
+        if (p.z > (R)g.zb[g.ksize]) p.z = (R)g.zb[g.ksize] - tiny;
+        int ix = 0, iy = 0, iz = 0;
+        while (ix < g.isize - 1 && (R)g.xb[ix + 1] < p.x) ix++;
+        while (iy < g.jsize - 1 && (R)g.yb[iy + 1] < p.y) iy++;
+        while (iz < g.ksize - 1 && (R)g.zb[iz + 1] < p.z) iz++;
+        p.ir = 1 + ix + iy * g.isize + iz * g.isize * g.jsize;
+    }
+    push<R>(p.iq == 0 ? qp : qe, p, st);
+}
+
+template <typename R>
+void Launch<R>::source(const DevJob *jobs, const DevPhantom<R> *ph, int job, uint64_t first, uint32_t n,
+                       const Queue<R> &qp, const Queue<R> &qe, DevStatus *st, hipStream_t s)
+{
+    if (!n) return;
+    hipLaunchKernelGGL(k_source<R>, dim3((n + 255) / 256), dim3(256), 0, s, jobs, ph, job, first, n, qp, qe, st);
+}
+
+template <typename R>
+void Launch<R>::photon(const KArgs<R> &a, int grid, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_photon<R>, dim3(grid), dim3(256), 0, s, a);
+}
+
+template <typename R>
+void Launch<R>::electron(const KArgs<R> &a, int grid, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_electron<R>, dim3(grid), dim3(256), 0, s, a);
+}
+
+}  // namespace omc

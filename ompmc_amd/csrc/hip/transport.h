@@ -1,0 +1,599 @@
+// Event-based transport: per-lane state machines for photons and charged
+// particles, and the kernels that drive them over structure-of-arrays
+// particle queues.
+//
+// ompMC runs each history depth-first through a per-thread stack
+// (shower(), ompmc.c:5436-5447).  Here a wavefront's 64 lanes each own one
+// particle of a single type; one loop trip advances every lane by one photon
+// voxel step or one electron ustep iteration, and a lane whose particle ends
+// refills itself from the input queue.  Secondaries go to the next
+// generation's queues.  Because every particle draws from its own Philox
+// stream, the order in which particles are processed does not change any
+// history.
+#pragma once
+
+#include "physics_e.h"
+
+namespace omc {
+
+constexpr double SGMFP = 1.0E-05;
+// Loop trips one particle may take within one launch before the lane gives
+// it up (counted in DevStatus::killed); far above any physical history.
+constexpr uint32_t MAX_ITERS = 1u << 22;
+
+template <typename R> struct KArgs {
+    DevTables<R> T;
+    const DevPhantom<R> *phantoms;
+    const DevJob *jobs;
+    Queue<R> qin;            // particles consumed by this launch
+    uint32_t in_begin, in_end;
+    uint32_t *fetch;         // fetch counter (starts at in_begin)
+    Queue<R> qp;             // photon output queue
+    Queue<R> qe;             // charged-particle output queue
+    DevStatus *status;
+};
+
+// Per-lane geometry / scoring context for the particle the lane holds.
+template <typename R> struct LaneGeo {
+    const DevPhantom<R> *g;
+    int isize, jsize, ksize;
+    const double *xb, *yb, *zb;
+    const uint8_t *med;
+    const R *rhof;
+    double *tally;           // job tally row of this history's batch
+    double *ledger;          // job ledger entry of this history (or null)
+
+    __device__ __forceinline__ void bind(const KArgs<R> &A, const Particle<R> &p)
+    {
+        const DevJob &J = A.jobs[p.job];
+        g = &A.phantoms[J.phantom];
+        isize = g->isize; jsize = g->jsize; ksize = g->ksize;
+        xb = g->xb; yb = g->yb; zb = g->zb;
+        med = g->med; rhof = g->rhof;
+        uint32_t batch = (uint32_t)((uint64_t)p.hist / (uint64_t)J.nperbatch);
+        tally = J.tally + (size_t)batch * (size_t)g->nreg;
+        ledger = J.ledger ? J.ledger + 3 * (size_t)p.hist : nullptr;
+    }
+    __device__ __forceinline__ int medium(int ir) const { int m = med[ir]; return m == 0xff ? -1 : m; }
+};
+
+// Deposit accumulator: consecutive deposits into the same region are summed
+// in a register and written with one FP64 atomic when the region changes.
+template <typename R> struct Scorer {
+    int ir = -1;
+    double acc = 0.0;
+    uint32_t ndep = 0;
+    __device__ __forceinline__ void flush(const LaneGeo<R> &G)
+    {
+        if (ir >= 0) {
+            atomicAdd(&G.tally[ir], acc);
+            if (G.ledger) atomicAdd(&G.ledger[ir > 0 ? 0 : 1], acc);
+        }
+        ir = -1;
+        acc = 0.0;
+    }
+    // ausgab (omc_dosxyz.c:683-694): wt * edep into region ir
+    __device__ __forceinline__ void add(const LaneGeo<R> &G, int region, double e)
+    {
+        ndep++;
+        if (region != ir) {
+            flush(G);
+            ir = region;
+        }
+        acc += e;
+    }
+};
+
+template <typename R>
+__device__ __forceinline__ void push(const Queue<R> &q, const Particle<R> &p, DevStatus *st)
+{
+    uint32_t i = wave_reserve(q.count, 1);
+    if (i < q.cap) q_store(q, i, p);
+    else atomicOr(&st->overflow, 1u);
+}
+
+// ============================================================ photons ======
+template <typename R> struct PhotonLane {
+    int phase;               // 0: (re)enter photon(), 1: transport step
+    int imed, lgle;
+    R gle, dpmfp, gmfp, cohfac, eig;
+};
+
+// Returns true while the photon is still owned by the lane.  Electrons it
+// sets in motion are pushed to A.qe.
+template <typename R>
+__device__ bool photon_iteration(const KArgs<R> &A, Particle<R> &p, PhotonLane<R> &s, LaneGeo<R> &G,
+                                 Scorer<R> &sc, uint32_t &nsteps)
+{
+    const DevTables<R> &T = A.T;
+    if (s.phase == 0) {
+        // photon() entry (ompmc.c:1884-1935) with nsplit = 1
+        int irl = p.ir;
+        s.imed = G.medium(irl);
+        s.eig = p.e;
+        if (p.e <= (R)G.g->pcut[s.imed + 1] || p.wt == (R)0) {
+            sc.add(G, irl, (double)p.wt * (double)p.e);
+            return false;
+        }
+        s.gle = m_log(p.e);
+        R rnno = urand<R>(p.rng);
+        rnno /= (R)1.0;
+        R d_eta = (R)1.0;
+        R eta_prime = (R)1.0 - rnno + d_eta;
+        (void)urand<R>(p.rng);          // survivor selection, trivial for nsplit = 1
+        eta_prime -= d_eta;
+        s.dpmfp = -m_log(eta_prime) - (R)0.0;
+        s.phase = 1;
+    }
+    // one trip of the transport loop (ompmc.c:1951-2019)
+    nsteps++;
+    int irl = p.ir;
+    R tstep;
+    if (s.imed != -1) {
+        const MediumScalars &M = T.small->med[s.imed];
+        s.lgle = pwlf_idx(s.gle, M.ge1, M.ge0) - 1;
+        R4<R> c = T.ph_mfp[s.imed * MXGE + s.lgle];
+        R gmfpr0 = s.gle * c.a1 + c.a0;
+        R rhof = G.rhof[irl];
+        s.gmfp = gmfpr0 / rhof;
+        s.cohfac = s.gle * c.b1 + c.b0;
+        s.gmfp *= s.cohfac;
+        tstep = s.gmfp * s.dpmfp;
+    } else {
+        tstep = (R)1.0E8;
+    }
+    int irnew = irl;
+    R ustep = tstep;
+    if (irl == 0) {                        // left the phantom: discard
+        if (G.ledger) atomicAdd(&G.ledger[2], (double)p.wt * (double)p.e);
+        return false;
+    }
+    howfar<R>(G, irl, p.x, p.y, p.z, p.u, p.v, p.w, irnew, ustep);
+    p.x += ustep * p.u;
+    p.y += ustep * p.v;
+    p.z += ustep * p.w;
+    if (s.imed != -1) s.dpmfp = m_fmax((R)0.0, s.dpmfp - ustep / s.gmfp);
+    if (irnew != irl) {
+        p.ir = irnew;
+        s.imed = G.medium(irnew);
+    }
+    if (!(s.imed != -1 && s.dpmfp <= (R)SGMFP)) return true;
+
+    // interaction (ompmc.c:2026-2068)
+    const int imed = s.imed;
+    R rnno = urand<R>(p.rng);
+    if (rnno <= (R)1.0 - s.cohfac) {
+        rayleigh<R>(T, p, imed, s.eig, s.gle, s.lgle);
+        s.phase = 0;
+        return true;
+    }
+    rnno = urand<R>(p.rng);
+    R4<R> br = T.ph_br[imed * MXGE + s.lgle];
+    R gbr1 = s.gle * br.a1 + br.a0;
+    Particle<R> b;
+    if (rnno <= gbr1 && s.eig > (R)2.0 * (R)RM) {
+        pair<R>(T, p, b, imed);
+        push<R>(A.qe, p, A.status);
+        push<R>(A.qe, b, A.status);
+        return false;
+    }
+    R gbr2 = s.gle * br.b1 + br.b0;
+    if (rnno < gbr2) {
+        compton<R>(p, b);
+        push<R>(A.qe, b, A.status);
+        s.phase = 0;
+        return true;
+    }
+    photo<R>(p, (R)G.g->ecut[imed + 1]);
+    push<R>(A.qe, p, A.status);
+    return false;
+}
+
+// ================================================== electrons/positrons ====
+template <typename R> struct ElectronLane {
+    int phase;               // 0 entry, 1 head of tstep loop, 2 ustep iteration
+    int imed, medold, lelke, compute_tstep;
+    R rhof, eke, elke, demfp, sig0, ustep;
+};
+
+enum { E_ALIVE = 0, E_DONE = 1 };
+
+// electron_cutoff: kinetic energy deposited; a positron then annihilates at
+// rest and the lane hands both photons to the photon queue.
+template <typename R>
+__device__ __forceinline__ void electron_cutoff(const KArgs<R> &A, Particle<R> &p, LaneGeo<R> &G,
+                                                Scorer<R> &sc, R eie)
+{
+    R edep = p.e - (R)RM;
+    sc.add(G, p.ir, (double)p.wt * (double)edep);
+    if (p.iq > 0 && edep < eie) {
+        Particle<R> b;
+        rannih<R>(p, b);
+        push<R>(A.qp, p, A.status);
+        push<R>(A.qp, b, A.status);
+    }
+}
+
+// One ustep iteration of electron() (ompmc.c:4765-5341) plus, where the
+// reference reaches them, the tstep-loop head (4694-4763), the fictitious
+// rejection (5347-5372) and the discrete interaction (5375-5429).
+template <typename R>
+__device__ int electron_iteration(const KArgs<R> &A, Particle<R> &p, ElectronLane<R> &s, LaneGeo<R> &G,
+                                  Scorer<R> &sc, uint32_t &nsteps)
+{
+    const DevTables<R> &T = A.T;
+    const R rm = (R)RM;
+    const int iq = p.iq;
+    const int qel = (1 + iq) / 2;
+    R eie = p.e;
+
+    if (s.phase == 0) {
+        int irl = p.ir;
+        s.imed = G.medium(irl);
+        s.rhof = G.rhof[irl];
+        s.medold = s.imed;
+        if (eie <= (R)G.g->ecut[s.imed + 1]) { electron_cutoff<R>(A, p, G, sc, eie); return E_DONE; }
+        s.phase = 1;
+    }
+    const MediumScalars *M = s.imed >= 0 ? &T.small->med[s.imed] : nullptr;
+    if (s.phase == 1) {
+        s.compute_tstep = 1;
+        s.eke = eie - rm;
+        if (s.imed != -1) {
+            R rnno = urand<R>(p.rng);
+            if (rnno == (R)0.0) rnno = (R)1.0E-30;
+            s.demfp = m_fmax(-m_log(rnno), (R)EPSEMFP);
+            s.elke = m_log(s.eke);
+            s.lelke = pwlf_idx(s.elke, M->eke1, M->eke0) - 1;
+            if (qel == 0 ? M->mono_e : M->mono_p) {
+                const ElecBin<R> &eb = T.el[(s.imed * 2 + qel) * MXEKE + s.lelke];
+                const ElecCsda &cs = T.csda[(s.imed * 2 + qel) * MXEKE + s.lelke];
+                s.sig0 = s.elke * eb.sig1 + eb.sig0;
+                R dedx0 = s.elke * (R)cs.dedx1 + (R)cs.dedx0;
+                s.sig0 /= dedx0;
+            } else {
+                s.sig0 = (R)(qel == 0 ? M->esig_e : M->psig_e);
+            }
+        }
+        s.phase = 2;
+    }
+
+    // ---- one ustep iteration ----
+    nsteps++;
+    int irl = p.ir;
+    int imed = s.imed;
+    bool call_howfar, do_single = false, called_msdist = false;
+    R tstep, tustep, range = 0, tvstep = 0, de = 0, tperp = 0;
+    double total_tstep = 0.0;             // fresh every iteration (ompmc.c:4787)
+    const ElecCsda *csb = nullptr;
+    R ustep;
+    R xf = 0, yf = 0, zf = 0, uf = 0, vf = 0, wf = 0;
+    if (imed == -1) {
+        tstep = (R)10.0E8;
+        ustep = tstep;
+        tustep = ustep;
+        call_howfar = true;
+    } else {
+        csb = T.csda + (imed * 2 + qel) * MXEKE;
+        const double eke0 = M->eke0, eke1 = M->eke1;
+        s.rhof = G.rhof[irl];
+        const R rhof = s.rhof;
+        if (s.sig0 <= (R)0.0) {
+            tstep = (R)10.0E8;
+            s.sig0 = (R)1.0E-15;
+        } else {
+            if (s.compute_tstep) {
+                double total_de = (double)s.demfp / (double)s.sig0;
+                double eke = (double)s.eke, elke = (double)s.elke;
+                double ekef = eke - total_de;
+                double ts;
+                if (ekef <= csb[0].earr) {
+                    ts = 10.0E8;
+                } else {
+                    double elkef = log(ekef);
+                    int lelkef = (int)(elkef * eke1 + eke0) - 1;
+                    if (lelkef == s.lelke) {
+                        ts = drange(&csb[s.lelke], iq, eke, ekef, elke, elkef);
+                    } else {
+                        double ekei = csb[s.lelke].earr;
+                        double elkei = (s.lelke + 1 - eke0) / eke1;
+                        double tuss = drange(&csb[s.lelke], iq, eke, ekei, elke, elkei);
+                        ekei = csb[lelkef + 1].earr;
+                        elkei = ((lelkef + 2) - eke0) / eke1;
+                        ts = drange(&csb[lelkef], iq, ekei, ekef, elkei, elkef);
+                        ts += tuss + csb[s.lelke].range - csb[lelkef + 1].range;
+                    }
+                }
+                total_tstep = ts;
+                s.compute_tstep = 0;
+            }
+            tstep = (R)(total_tstep / (double)rhof);
+        }
+        const ElecBin<R> &eb = T.el[(imed * 2 + qel) * MXEKE + s.lelke];
+        const ElecCsda &cs = csb[s.lelke];
+        R dedx0 = s.elke * (R)cs.dedx1 + (R)cs.dedx0;
+        R dedx = rhof * dedx0;
+        R tmxs = s.elke * eb.tmxs1 + eb.tmxs0;
+        tmxs /= rhof;
+        {
+            double ekei = cs.earr;
+            double elkei = (s.lelke + 1 - eke0) / eke1;
+            double rg = drange(&cs, iq, (double)s.eke, ekei, (double)s.elke, elkei);
+            rg += cs.range;
+            rg /= (double)rhof;
+            range = (R)rg;
+        }
+        tustep = m_fmin(m_fmin(tstep, tmxs), range);
+        tperp = hownear<R>(G, irl, p.x, p.y, p.z);
+        R blccl = rhof * (R)M->blcc;
+        R xccl = rhof * (R)M->xcc;
+        R p2 = s.eke * (s.eke + (R)2.0 * rm);
+        R beta2 = p2 / (p2 + sq(rm));
+        R etap = s.elke * eb.eta1 + eb.eta0;
+        R ms_corr = s.elke * eb.blc1 + eb.blc0;
+        blccl = blccl / etap / ((R)1.0 + (R)0.25 * etap * xccl / blccl / p2) * ms_corr;
+        R ssmfp = beta2 / blccl;
+        R skindepth = (R)3 * ssmfp;
+        tustep = m_fmin(tustep, m_fmax(tperp, skindepth));
+        if ((tustep <= tperp) && (tustep > skindepth)) {
+            call_howfar = false;
+            do_single = false;
+            called_msdist = true;
+            de = (R)eloss(csb, M->te, eke0, eke1, (double)rhof, (double)tustep, (double)range,
+                          (double)s.eke, (double)s.elke, s.lelke);
+            tvstep = tustep;
+            // msdist moves a copy; the original position is kept until
+            // after ausgab, which scores in the pre-step region
+            Particle<R> q = p;
+            ustep = msdist<R>(T, q, imed, rhof, de, tustep, s.eke);
+            p.rng = q.rng;
+            xf = q.x; yf = q.y; zf = q.z; uf = q.u; vf = q.v; wf = q.w;
+        } else {
+            called_msdist = false;
+            R rnno = urand<R>(p.rng);
+            if (rnno < (R)1.0E-30) rnno = (R)1.0E-30;
+            R lambda = (R)(-1.0) * m_log((R)1.0 - rnno);
+            R lambda_max = (R)0.5 * blccl * rm / dedx;
+            lambda_max *= (s.eke / rm + (R)1.0) * (s.eke / rm + (R)1.0) * (s.eke / rm + (R)1.0);
+            if (lambda >= (R)0.0 && lambda_max > (R)0.0) {
+                R tuss;
+                if (lambda < lambda_max) tuss = lambda * ssmfp * ((R)1.0 - (R)0.5 * lambda / lambda_max);
+                else tuss = (R)0.5 * lambda * ssmfp;
+                if (tuss < tustep) { tustep = tuss; do_single = true; }
+                else do_single = false;
+            } else {
+                return E_DONE;               // reference drops the particle unscored
+            }
+            ustep = tustep;
+            call_howfar = !(ustep < tperp);
+        }
+    }
+
+    int irold = p.ir, irnew = p.ir;
+    if (call_howfar) {
+        if (irl == 0) {                      // howfar discard (outside)
+            R edep = (iq > 0) ? p.e + rm : p.e - rm;
+            sc.add(G, p.ir, (double)p.wt * (double)edep);
+            if (iq > 0 && edep < eie) {
+                Particle<R> b;
+                rannih<R>(p, b);
+                push<R>(A.qp, p, A.status);
+                push<R>(A.qp, b, A.status);
+            }
+            return E_DONE;
+        }
+        howfar<R>(G, irl, p.x, p.y, p.z, p.u, p.v, p.w, irnew, ustep);
+    }
+    if (ustep < (R)0) ustep = (R)0.0;
+    s.ustep = ustep;
+    if (ustep == (R)0.0 || imed == -1) {
+        if (ustep != (R)0.0) {
+            R vstep = ustep;
+            p.x += p.u * vstep;
+            p.y += p.v * vstep;
+            p.z += p.w * vstep;
+        }
+        if (irnew != irold) {
+            p.ir = irnew;
+            irl = irnew;
+            s.imed = G.medium(irl);
+        }
+        if (eie <= (R)G.g->ecut[s.imed + 1]) { electron_cutoff<R>(A, p, G, sc, eie); return E_DONE; }
+        s.phase = 1;                         // break -> continue tstep loop
+        return E_ALIVE;
+    }
+    R vstep = ustep;
+    const R rhof = s.rhof;
+    if (call_howfar) {
+        tvstep = vstep;
+        if (tvstep != tustep) do_single = false;
+        de = (R)eloss(csb, M->te, M->eke0, M->eke1, (double)rhof, (double)tvstep, (double)range,
+                      (double)s.eke, (double)s.elke, s.lelke);
+    } else {
+        tvstep = tustep;
+        if (!called_msdist)
+            de = (R)eloss(csb, M->te, M->eke0, M->eke1, (double)rhof, (double)tvstep, (double)range,
+                          (double)s.eke, (double)s.elke, s.lelke);
+    }
+    R edep = de;
+    R ekef = s.eke - de;
+    if (!called_msdist) {
+        R sinthe, costhe;
+        if (do_single) {
+            R ekems = m_fmax(ekef, (R)G.g->ecut[imed + 1] - rm);
+            R p2 = ekems * (ekems + (R)2.0 * rm);
+            R beta2 = p2 / (p2 + sq(rm));
+            R chia2 = (R)M->xcc / ((R)4.0 * (R)M->blcc * p2);
+            R elkems = m_log(ekems);
+            int lelkems = pwlf_idx(elkems, M->eke1, M->eke0) - 1;
+            const ElecBin<R> &ebs = T.el[(imed * 2 + qel) * MXEKE + lelkems];
+            R etap = elkems * ebs.eta1 + ebs.eta0;
+            chia2 *= etap;
+            sscat<R>(T, p.rng, imed, qel, chia2, elkems, beta2, costhe, sinthe);
+        } else {
+            sinthe = (R)0.0;
+            costhe = (R)1.0;
+        }
+        xf = p.x + p.u * vstep;
+        yf = p.y + p.v * vstep;
+        zf = p.z + p.w * vstep;
+        if (do_single) {
+            Uphi<R> h;
+            uphi21<R>(p, h, costhe, sinthe);
+        }
+        uf = p.u; vf = p.v; wf = p.w;
+    }
+    sc.add(G, p.ir, (double)p.wt * (double)edep);
+    p.x = xf; p.y = yf; p.z = zf;
+    p.u = uf; p.v = vf; p.w = wf;
+    irold = p.ir;
+    eie -= edep;
+    p.e = eie;
+    if (irnew == irl && eie <= (R)G.g->ecut[imed + 1]) { electron_cutoff<R>(A, p, G, sc, eie); return E_DONE; }
+    s.medold = imed;
+    if (imed != -1) {
+        s.eke = eie - rm;
+        s.elke = m_log(s.eke);
+        s.lelke = pwlf_idx(s.elke, M->eke1, M->eke0) - 1;
+    }
+    if (irnew != irold) {
+        p.ir = irnew;
+        irl = irnew;
+        s.imed = G.medium(irl);
+    }
+    if (eie <= (R)G.g->ecut[s.imed + 1]) { electron_cutoff<R>(A, p, G, sc, eie); return E_DONE; }
+    if (s.imed != s.medold) { s.phase = 1; return E_ALIVE; }
+    s.demfp -= de * s.sig0;
+    total_tstep -= (double)tvstep * (double)rhof;
+    if (total_tstep < 1.0E-9) s.demfp = (R)0.0;
+    if (s.demfp >= (R)EPSEMFP) return E_ALIVE;      // next ustep iteration
+
+    // ---- end of the ustep loop: fictitious-cross-section rejection ----
+    {
+        const ElecBin<R> &eb = T.el[(s.imed * 2 + qel) * MXEKE + s.lelke];
+        const ElecCsda &cs = T.csda[(s.imed * 2 + qel) * MXEKE + s.lelke];
+        R sigf = s.elke * eb.sig1 + eb.sig0;
+        R dedx0 = s.elke * (R)cs.dedx1 + (R)cs.dedx0;
+        sigf /= dedx0;
+        R sigratio = sigf / s.sig0;
+        R rfict = urand<R>(p.rng);
+        if (rfict >= sigratio) { s.phase = 1; return E_ALIVE; }
+    }
+
+    // ---- discrete interaction ----
+    const int m = s.imed;
+    const ElecBin<R> &eb = T.el[(m * 2 + qel) * MXEKE + s.lelke];
+    Particle<R> b;
+    s.phase = 0;
+    if (iq < 0) {
+        R ebr1 = s.elke * eb.br11 + eb.br10;
+        R rnno = urand<R>(p.rng);
+        if (rnno <= ebr1) {
+            brems<R>(T, p, b, m);
+            push<R>(A.qp, b, A.status);
+        } else if (p.e <= (R)T.small->med[m].thmoll) {
+            if (ebr1 <= (R)0) return E_ALIVE;   // neither process possible: re-enter
+            brems<R>(T, p, b, m);
+            push<R>(A.qp, b, A.status);
+        } else {
+            if (moller<R>(T, p, b, m)) push<R>(A.qe, b, A.status);
+        }
+        return E_ALIVE;
+    }
+    R pbr1 = s.elke * eb.br11 + eb.br10;
+    R rnno = urand<R>(p.rng);
+    if (rnno < pbr1) {
+        brems<R>(T, p, b, m);
+        push<R>(A.qp, b, A.status);
+        return E_ALIVE;
+    }
+    R pbr2 = s.elke * eb.br21 + eb.br20;
+    if (rnno < pbr2) {
+        bhabha<R>(T, p, b, m);
+        push<R>(A.qe, b, A.status);
+        return E_ALIVE;
+    }
+    annih<R>(p, b);
+    push<R>(A.qp, p, A.status);
+    push<R>(A.qp, b, A.status);
+    return E_DONE;
+}
+
+// ============================================================ kernels =====
+template <typename R>
+__global__ void __launch_bounds__(256) k_photon(KArgs<R> A)
+{
+    Particle<R> p;
+    PhotonLane<R> s;
+    LaneGeo<R> G;
+    Scorer<R> sc;
+    bool alive = false, exhausted = false;
+    uint32_t nsteps = 0, iters = 0;
+    for (;;) {
+        if (!alive && !exhausted) {
+            uint32_t i = wave_reserve(A.fetch, 1);
+            if (i < A.in_end) {
+                q_load(A.qin, i, p);
+                G.bind(A, p);
+                s.phase = 0;
+                alive = true;
+            } else {
+                exhausted = true;
+            }
+        }
+        if (__ballot(alive) == 0) break;
+        if (alive) {
+            alive = photon_iteration<R>(A, p, s, G, sc, nsteps);
+            if (alive && ++iters > MAX_ITERS) {         // watchdog: no lane runs forever
+                atomicAdd(&A.status->killed, 1u);
+                alive = false;
+            }
+            if (!alive) { sc.flush(G); iters = 0; }
+        }
+    }
+    sc.flush(G);
+    atomicAdd(&A.status->photon_steps, (unsigned long long)nsteps);
+    atomicAdd(&A.status->deposits, (unsigned long long)sc.ndep);
+}
+
+template <typename R>
+__global__ void __launch_bounds__(256) k_electron(KArgs<R> A)
+{
+    Particle<R> p;
+    ElectronLane<R> s;
+    LaneGeo<R> G;
+    Scorer<R> sc;
+    bool alive = false, exhausted = false;
+    uint32_t nsteps = 0, iters = 0;
+    for (;;) {
+        if (!alive && !exhausted) {
+            uint32_t i = wave_reserve(A.fetch, 1);
+            if (i < A.in_end) {
+                q_load(A.qin, i, p);
+                G.bind(A, p);
+                s.phase = 0;
+                alive = true;
+            } else {
+                exhausted = true;
+            }
+        }
+        if (__ballot(alive) == 0) break;
+        if (alive) {
+            int r = electron_iteration<R>(A, p, s, G, sc, nsteps);
+            if (r != E_DONE && ++iters > MAX_ITERS) {   // watchdog: no lane runs forever
+                atomicAdd(&A.status->killed, 1u);
+                r = E_DONE;
+            }
+            if (r == E_DONE) {
+                alive = false;
+                sc.flush(G);
+                iters = 0;
+            }
+        }
+    }
+    sc.flush(G);
+    atomicAdd(&A.status->electron_steps, (unsigned long long)nsteps);
+    atomicAdd(&A.status->deposits, (unsigned long long)sc.ndep);
+}
+
+}  // namespace omc

@@ -1,0 +1,116 @@
+"""ctypes binding of libomc_hip.so (include/omc_dev.h) for tests and the
+benchmark.  Loading fails loudly when the HIP library has not been built;
+there is no CPU fallback."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _native as N
+
+FP64, FP32 = 0, 1
+_dp = C.POINTER(C.c_double)
+
+
+class DevOpts(C.Structure):
+    _fields_ = [("device", C.c_int), ("precision", C.c_int), ("hbm_budget", C.c_int64), ("chunk", C.c_int64)]
+
+
+class DevStats(C.Structure):
+    _fields_ = [("histories", C.c_int64), ("photon_steps", C.c_int64), ("electron_steps", C.c_int64),
+                ("deposits", C.c_int64), ("photons_pushed", C.c_int64), ("electrons_pushed", C.c_int64),
+                ("generations", C.c_int64), ("chunks", C.c_int64), ("transport_ms", C.c_double),
+                ("photon_kernel_ms", C.c_double), ("electron_kernel_ms", C.c_double),
+                ("source_kernel_ms", C.c_double), ("photon_launches", C.c_int64),
+                ("electron_launches", C.c_int64), ("killed", C.c_int64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class RunParams(C.Structure):
+    _fields_ = [("phantom_id", C.c_int), ("nsplit", C.c_int), ("seed", C.c_uint32 * 2), ("hist0", C.c_int64),
+                ("nperbatch", C.c_int64), ("nbatch", C.c_int), ("ledger", _dp),
+                ("stats", C.POINTER(DevStats))]
+
+
+class Request(C.Structure):
+    _fields_ = [("phantom_id", C.c_int), ("kind", C.c_int), ("src_dosxyz", C.POINTER(N.SourceDosxyz)),
+                ("src_beamlets", C.POINTER(N.SourceBeamlets)), ("beamlet", C.c_int), ("nhist", C.c_int64),
+                ("nbatch", C.c_int), ("seed", C.c_uint32 * 2)]
+
+
+class Result(C.Structure):
+    _fields_ = [("ticket", C.c_int), ("done", C.c_int), ("submit_ms", C.c_double), ("admit_ms", C.c_double),
+                ("finish_ms", C.c_double), ("nhist", C.c_int64)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        path = N.LIBDIR / "libomc_hip.so"
+        if not path.exists():
+            raise RuntimeError(f"{path} is not built; the device path has no fallback")
+        N.host()
+        L = C.CDLL(str(path))
+        L.omc_dev_last_error.restype = C.c_char_p
+        L.omc_dev_create.argtypes = [C.POINTER(C.c_void_p), C.POINTER(DevOpts)]
+        L.omc_dev_destroy.argtypes = [C.c_void_p]
+        L.omc_dev_upload_media.argtypes = [C.c_void_p, C.POINTER(N.Tables)]
+        L.omc_dev_upload_phantom.argtypes = [C.c_void_p, C.POINTER(N.Phantom), C.POINTER(N.Regions),
+                                             C.POINTER(C.c_int)]
+        L.omc_dev_release_phantom.argtypes = [C.c_void_p, C.c_int]
+        L.omc_dev_run_dosxyz.argtypes = [C.c_void_p, C.POINTER(N.SourceDosxyz), C.POINTER(RunParams), _dp, _dp]
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc < 0:
+        raise RuntimeError(f"omc_dev error {rc}: {lib().omc_dev_last_error().decode()}")
+    return rc
+
+
+class Engine:
+    """One device context: tables + resident phantoms."""
+
+    def __init__(self, precision=FP64, device=0, hbm_budget=0, chunk=0):
+        L = lib()
+        self.ctx = C.c_void_p()
+        opts = DevOpts(device, precision, hbm_budget, chunk)
+        check(L.omc_dev_create(C.byref(self.ctx), C.byref(opts)))
+        self.precision = precision
+
+    def close(self):
+        if self.ctx:
+            lib().omc_dev_destroy(self.ctx)
+            self.ctx = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def upload_media(self, tables):
+        check(lib().omc_dev_upload_media(self.ctx, C.byref(tables)))
+
+    def upload_phantom(self, phantom, regions) -> int:
+        pid = C.c_int(-1)
+        check(lib().omc_dev_upload_phantom(self.ctx, C.byref(phantom), C.byref(regions), C.byref(pid)))
+        return pid.value
+
+    def run_dosxyz(self, pid, src, nperbatch, nbatch, seed=(97, 33), hist0=0, nreg=None, ledger=False):
+        stats = DevStats()
+        led = np.zeros(3 * nperbatch * nbatch) if ledger else None
+        rp = RunParams(pid, 1, (C.c_uint32 * 2)(*seed), hist0, nperbatch, nbatch,
+                       led.ctypes.data_as(_dp) if ledger else None, C.pointer(stats))
+        s = np.zeros(nreg)
+        s2 = np.zeros(nreg)
+        check(lib().omc_dev_run_dosxyz(self.ctx, C.byref(src), C.byref(rp), s.ctypes.data_as(_dp),
+                                       s2.ctypes.data_as(_dp)))
+        return s, s2, stats, led
