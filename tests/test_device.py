@@ -1,0 +1,227 @@
+"""Device engine (libomc_hip.so, through its C ABI) against the oracle.
+
+Replay parity: the device and the oracle in PHILOX mode draw every particle's
+random numbers from the same per-particle Philox stream, so with the same
+seed they follow the same histories; FP64 results must agree to rounding.
+The oracle's RANMAR mode is bit-identical to the reference, and the Philox
+mode is statistically identical to it (test_oracle_reference.py), which
+closes the chain device -> oracle -> reference.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+from cases import write_case
+
+pytestmark = pytest.mark.gpu
+
+RM = 0.5109989461
+
+
+@pytest.fixture(scope="module")
+def D():
+    from ompmc_amd import device
+    return device
+
+
+def run_both(oracle, D, stem, seed=(21, 5), precision=0, chunk=0):
+    prob = oracle.load_problem(stem)
+    try:
+        so, so2, cnt = oracle.run_dosxyz_philox(prob, seed=seed, nthreads=1)
+        L = oracle.lib()
+        ctx = L.ora_create(C.byref(prob.tab), C.byref(prob.reg), C.byref(prob.ph), 1, oracle.PHILOX)
+        L.ora_seed_philox(ctx, *seed)
+        led_o = np.zeros(3 * prob.nhist)
+        L.ora_set_ledger(ctx, led_o.ctypes.data_as(C.POINTER(C.c_double)), prob.nhist)
+        t1, t2 = np.zeros(prob.reg.nreg), np.zeros(prob.reg.nreg)
+        P = C.POINTER(C.c_double)
+        assert L.ora_run_dosxyz(ctx, C.byref(prob.src), prob.nperbatch, prob.nbatch, t1.ctypes.data_as(P),
+                                t2.ctypes.data_as(P)) == 0
+        L.ora_destroy(ctx)
+        with D.Engine(precision, chunk=chunk) as eng:
+            eng.upload_media(prob.tab)
+            pid = eng.upload_phantom(prob.ph, prob.reg)
+            sd, sd2, st, led_d = eng.run_dosxyz(pid, prob.src, prob.nperbatch, prob.nbatch, seed=seed,
+                                                nreg=prob.reg.nreg, ledger=True)
+        return dict(so=so, so2=so2, sd=sd, sd2=sd2, st=st, led_o=led_o.reshape(-1, 3),
+                    led_d=led_d.reshape(-1, 3), nb=prob.nbatch, nhist=prob.nhist)
+    finally:
+        oracle.free_problem(prob)
+
+
+REPLAY = ["water2MeV", "slab6MV", "patient6MV", "electron6MeV", "positron6MeV", "pencil", "oversize"]
+
+
+@pytest.mark.parametrize("case", REPLAY)
+def test_fp64_replays_oracle_histories(oracle, D, tmp_path, case):
+    r = run_both(oracle, D, write_case(case, tmp_path, ncase=1000))
+    so, sd = r["so"], r["sd"]
+    scale = np.abs(so).max()
+    close = np.abs(sd - so) <= 1e-9 * scale
+    assert close.mean() >= 0.999, f"{case}: {np.count_nonzero(~close)} voxels differ"
+    same = np.all(np.abs(r["led_d"] - r["led_o"]) <= 1e-9 * (1 + np.abs(r["led_o"])), axis=1)
+    assert same.mean() >= 0.995, f"{case}: {np.count_nonzero(~same)} histories differ"
+    assert abs(sd.sum() / so.sum() - 1) < 1e-6
+    assert r["st"].killed == 0
+
+
+@pytest.mark.parametrize("case,e_in", [("water2MeV", 2.0), ("electron6MeV", 6.0), ("positron6MeV", 6.0 + 2 * RM)])
+@pytest.mark.parametrize("precision,tol", [(0, 1e-12), (1, 2e-6)])
+def test_energy_conservation_per_history(oracle, D, tmp_path, case, e_in, precision, tol):
+    stem = write_case(case, tmp_path, ncase=2000)
+    prob = oracle.load_problem(stem)
+    try:
+        with D.Engine(precision) as eng:
+            eng.upload_media(prob.tab)
+            pid = eng.upload_phantom(prob.ph, prob.reg)
+            s, _, st, led = eng.run_dosxyz(pid, prob.src, prob.nperbatch, prob.nbatch, nreg=prob.reg.nreg,
+                                           ledger=True)
+    finally:
+        oracle.free_problem(prob)
+    tot = led.reshape(-1, 3).sum(1)
+    assert np.allclose(tot, e_in, rtol=tol, atol=tol)
+    assert abs(led.reshape(-1, 3)[:, :2].sum() - s.sum()) < 1e-9 * s.sum()
+
+
+def _stat(m1, e1, m2, e2):
+    sel = np.where(m1[1:] > 0.1 * m1[1:].max())[0] + 1
+    z = (m1[sel] - m2[sel]) / np.sqrt(e1[sel] ** 2 + e2[sel] ** 2 + 1e-300)
+    return (np.abs(z) < 2).mean(), len(sel)
+
+
+def _mv(s, s2, nb):
+    m = s / nb
+    return m, np.sqrt(np.maximum(s2 / nb - m * m, 0) / (nb - 1))
+
+
+@pytest.mark.parametrize("case", ["slab6MV", "patient6MV", "electron6MeV"])
+def test_fp32_agrees_with_fp64_within_2_sigma(oracle, D, tmp_path, case):
+    """FP32 transport with FP64 tallies against the FP64 kernel (different
+    rounding, so independent histories after the first divergence): >= 95%
+    of voxels above 10% Dmax within 2 combined sigma (50 batches), integral
+    energy within 0.5%."""
+    stem = write_case(case, tmp_path, ncase=200000)
+    inp = tmp_path / f"{case}.inp"
+    inp.write_text(inp.read_text().replace("nbatch = 10", "nbatch = 50"))
+    prob = oracle.load_problem(stem)
+    try:
+        res = {}
+        for prec in (0, 1):
+            with D.Engine(prec) as eng:
+                eng.upload_media(prob.tab)
+                pid = eng.upload_phantom(prob.ph, prob.reg)
+                res[prec] = eng.run_dosxyz(pid, prob.src, prob.nperbatch, prob.nbatch, seed=(8, 1 + prec),
+                                           nreg=prob.reg.nreg)
+        nb = prob.nbatch
+    finally:
+        oracle.free_problem(prob)
+    m64, e64 = _mv(res[0][0], res[0][1], nb)
+    m32, e32 = _mv(res[1][0], res[1][1], nb)
+    frac, n = _stat(m64, e64, m32, e32)
+    assert n > 50 and frac >= 0.95, f"{case}: {frac:.3f} of {n} voxels within 2 sigma"
+    assert abs(res[1][0][1:].sum() / res[0][0][1:].sum() - 1) < 0.005
+
+
+def test_device_matches_reference_rng_statistically(oracle, D, tmp_path):
+    """Device FP64 (Philox streams) against the oracle in RANMAR mode, which
+    is bit-identical to the reference: same dose distribution."""
+    stem = write_case("slab6MV", tmp_path, ncase=200000)
+    inp = tmp_path / "slab6MV.inp"
+    inp.write_text(inp.read_text().replace("nbatch = 10", "nbatch = 50"))
+    prob = oracle.load_problem(stem)
+    try:
+        r, r2, _ = oracle.run_dosxyz_ranmar(prob)
+        with D.Engine(0) as eng:
+            eng.upload_media(prob.tab)
+            pid = eng.upload_phantom(prob.ph, prob.reg)
+            d, d2, _, _ = eng.run_dosxyz(pid, prob.src, prob.nperbatch, prob.nbatch, nreg=prob.reg.nreg)
+        nb = prob.nbatch
+    finally:
+        oracle.free_problem(prob)
+    mr, er = _mv(r, r2, nb)
+    md, ed = _mv(d, d2, nb)
+    frac, n = _stat(mr, er, md, ed)
+    assert frac >= 0.93, f"{frac:.3f} of {n}"
+    assert abs(d[1:].sum() / r[1:].sum() - 1) < 0.01
+
+
+def test_chunking_and_reruns_do_not_change_histories(oracle, D, tmp_path):
+    """Histories are identified by (seed, index) only: splitting the run into
+    small chunks, or repeating it, reproduces the same energies up to FP64
+    summation order."""
+    stem = write_case("patient6MV", tmp_path, ncase=5000)
+    prob = oracle.load_problem(stem)
+    try:
+        out = []
+        for chunk in (0, 0, 777):
+            with D.Engine(0, chunk=chunk) as eng:
+                eng.upload_media(prob.tab)
+                pid = eng.upload_phantom(prob.ph, prob.reg)
+                s, _, st, led = eng.run_dosxyz(pid, prob.src, prob.nperbatch, prob.nbatch, seed=(4, 4),
+                                               nreg=prob.reg.nreg, ledger=True)
+                out.append((s, led, st.chunks))
+    finally:
+        oracle.free_problem(prob)
+    assert out[2][2] > 1
+    for s, led, _ in out[1:]:
+        assert np.allclose(s, out[0][0], rtol=1e-12, atol=1e-12 * out[0][0].max())
+        assert np.allclose(led, out[0][1], rtol=1e-12, atol=1e-12)
+
+
+def test_vacuum_voxels_inside_phantom(oracle, D, tmp_path):
+    """Medium 0 voxels (vacuum) inside the grid: straight-line transport,
+    same histories as the oracle."""
+    from ompmc_amd import configs as cfg
+    ph = cfg.water_cube(10, 5.0)
+    ph.med[4:6, 3:7, 3:7] = 0
+    ph.rho[4:6, 3:7, 3:7] = 0.0
+    pf = ph.write_egsphant(tmp_path / "vac.egsphant")
+    items = cfg.dosxyz_inp_items(tmp_path, pf, ncase=1000, mono=3.0, field=(-1, 1, -1, 1))
+    cfg.write_inp(tmp_path / "vac", items)
+    r = run_both(oracle, D, tmp_path / "vac")
+    scale = np.abs(r["so"]).max()
+    assert (np.abs(r["sd"] - r["so"]) <= 1e-9 * scale).mean() >= 0.999
+
+
+def test_error_paths_return_codes(oracle, D, tmp_path):
+    stem = write_case("water2MeV", tmp_path, ncase=100)
+    prob = oracle.load_problem(stem)
+    try:
+        with D.Engine(0) as eng:
+            with pytest.raises(RuntimeError, match="media"):
+                eng.upload_phantom(prob.ph, prob.reg)
+            eng.upload_media(prob.tab)
+            pid = eng.upload_phantom(prob.ph, prob.reg)
+            L = D.lib()
+            st = D.DevStats()
+            s = np.zeros(prob.reg.nreg)
+            P = C.POINTER(C.c_double)
+            rp = D.RunParams(pid, 5, (C.c_uint32 * 2)(1, 2), 0, 10, 2, None, C.pointer(st))
+            assert L.omc_dev_run_dosxyz(eng.ctx, C.byref(prob.src), C.byref(rp), s.ctypes.data_as(P),
+                                        s.ctypes.data_as(P)) == -10          # nsplit > 1 unsupported
+            rp = D.RunParams(pid + 7, 1, (C.c_uint32 * 2)(1, 2), 0, 10, 2, None, C.pointer(st))
+            assert L.omc_dev_run_dosxyz(eng.ctx, C.byref(prob.src), C.byref(rp), s.ctypes.data_as(P),
+                                        s.ctypes.data_as(P)) == -4           # unknown phantom
+            rp = D.RunParams(pid, 1, (C.c_uint32 * 2)(1, 2), 0, 0, 2, None, C.pointer(st))
+            assert L.omc_dev_run_dosxyz(eng.ctx, C.byref(prob.src), C.byref(rp), s.ctypes.data_as(P),
+                                        s.ctypes.data_as(P)) == -4           # empty batch
+        with pytest.raises(RuntimeError, match="budget"):
+            D.Engine(0, hbm_budget=1 << 20)
+    finally:
+        oracle.free_problem(prob)
+
+
+def test_single_history_and_single_batch(oracle, D, tmp_path):
+    stem = write_case("electron6MeV", tmp_path, ncase=1)
+    prob = oracle.load_problem(stem)
+    try:
+        so, _, _ = oracle.run_dosxyz_philox(prob, seed=(1, 1), nperbatch=1, nbatch=1, nthreads=1)
+        with D.Engine(0) as eng:
+            eng.upload_media(prob.tab)
+            pid = eng.upload_phantom(prob.ph, prob.reg)
+            sd, sd2, st, _ = eng.run_dosxyz(pid, prob.src, 1, 1, seed=(1, 1), nreg=prob.reg.nreg)
+    finally:
+        oracle.free_problem(prob)
+    assert st.histories == 1
+    assert np.allclose(sd, so, rtol=1e-12, atol=1e-12) and np.allclose(sd2, sd * sd, rtol=1e-12, atol=1e-15)
