@@ -40,7 +40,10 @@ typedef struct {
     int     device;          /* HIP device ordinal */
     int     precision;       /* OMC_PREC_FP64 / OMC_PREC_FP32 */
     int64_t hbm_budget;      /* bytes the context may hold; 0 = 90% of free HBM */
-    int64_t chunk;           /* primaries per transport chunk; 0 = automatic */
+    int64_t chunk;           /* live particles per pipeline before new primaries
+                                are held back; 0 = automatic */
+    int     pipelines;       /* concurrent transport pipelines (streams);
+                                0 = automatic */
 } omc_dev_opts;
 
 /* Device counters of one run (all sums over the run). */
@@ -50,16 +53,21 @@ typedef struct {
     int64_t electron_steps;     /* electron ustep iterations */
     int64_t deposits;           /* ausgab calls */
     int64_t photons_pushed, electrons_pushed;
-    int64_t generations;
-    int64_t chunks;
-    double  transport_ms;       /* device time of the transport kernels */
-    double  photon_kernel_ms, electron_kernel_ms, source_kernel_ms;
+    int64_t generations;        /* photon+electron launch pairs, all pipelines */
+    int64_t injections;         /* source launches (primaries handed to a pipeline) */
+    double  transport_ms;       /* wall time of the run on the engine clock */
+    double  photon_kernel_ms, electron_kernel_ms, source_kernel_ms;   /* sums of
+                                   launch durations; pipelines overlap, so these
+                                   may add up to more than transport_ms */
     int64_t photon_launches, electron_launches;
     int64_t killed;             /* particles stopped by the step-limit watchdog */
+    int64_t photon_crossings, electron_crossings;   /* voxel boundaries crossed */
+    int64_t photon_deposits, electron_deposits;     /* ausgab calls per kernel */
 } omc_dev_stats;
 
-/* Context: device, stream, HBM budget, precision.  Replaces the per-thread
- * initStack/initRandom set-up (ompmc.c:47-64, omc_random.c:58-119). */
+/* Context: device, pipelines (each a stream with its own particle queues),
+ * HBM budget, precision.  Replaces the per-thread initStack/initRandom
+ * set-up (ompmc.c:47-64, omc_random.c:58-119). */
 int  omc_dev_create(omc_dev_ctx **ctx, const omc_dev_opts *opts);
 int  omc_dev_destroy(omc_dev_ctx *ctx);
 const char *omc_dev_last_error(void);
@@ -98,21 +106,27 @@ typedef struct {
 int  omc_dev_run_dosxyz(omc_dev_ctx *ctx, const omc_source_dosxyz *src,
                         const omc_dev_run_params *rp, double *sum, double *sum2);
 
-/* matRad dose influence: every beamlet of `src` in one device pass
- * (omc_matrad.c:1389-1493).  nhist[b] histories for beamlet b split into
- * rp->nbatch batches.  Outputs per beamlet b (caller arrays):
- *   sum [b*nreg + i], sum2[b*nreg + i]   batch sums (may be NULL if only
- *                                        the sparse output is wanted)
- * The sparse thresholded Dij is produced by omc_dev_beamlets_sparse(). */
+/* matRad dose influence: every beamlet of `src` in one device pass instead
+ * of the serial beamlet loop (omc_matrad.c:1389-1414).  Beamlet b runs
+ * nperbatch_b * rp->nbatch histories with nperbatch_b = nhist[b] / nbatch
+ * (1 if that is 0, as omc_matrad.c:1332-1337 does for its single count);
+ * its histories take stream indices after those of beamlets 0..b-1,
+ * starting at rp->hist0.  rp->nperbatch and rp->ledger are not used.
+ * sum [b*nreg + i] and sum2[b*nreg + i] are increased by the batch sums,
+ * as omc_dev_run_dosxyz does for one field. */
 int  omc_dev_run_beamlets(omc_dev_ctx *ctx, const omc_source_beamlets *src,
                           const int64_t *nhist, const omc_dev_run_params *rp,
                           double *sum, double *sum2);
 
 /* Same beamlet pass, with the matRad Dij assembled on the device: for each
- * beamlet the mean dose per history (omc_matrad.c:1418 normalisation by
- * the beamlet's history count) above rel_thr * beamlet max is emitted as one
- * CSC column.  jcs has nbeamlets+1 entries; irs/vals have `cap` entries;
- * *nnz receives the number written (OMC_ERANGE if cap is too small). */
+ * beamlet the dose per incident history (accumulateResults with iout = 1,
+ * omc_matrad.c:839-918, normalised by the beamlet's own history count)
+ * above rel_thr * beamlet max becomes one CSC column, rows in voxel order
+ * (omc_matrad.c:1416-1479).  jcs has nbeamlets+1 entries; irs/vals have
+ * `cap` entries; *nnz receives the total.  When it exceeds cap the call
+ * returns OMC_ERANGE, writes nothing and keeps the matrix in the context:
+ * a call with src == NULL (nhist and rp unused) then copies the kept
+ * matrix into larger arrays without running the beamlets again. */
 int  omc_dev_beamlets_sparse(omc_dev_ctx *ctx, const omc_source_beamlets *src,
                              const int64_t *nhist, const omc_dev_run_params *rp,
                              double rel_thr, int64_t *jcs, int64_t *irs,
@@ -132,20 +146,26 @@ typedef struct {
 
 typedef struct {
     int     ticket;
-    int     done;               /* 1 when finished */
+    int     done;               /* 1 when finished (or failed) */
+    int     status;             /* OMC_OK, or the error that ended it */
     double  submit_ms, admit_ms, finish_ms;   /* engine clock */
-    int64_t nhist;
+    int64_t nhist;              /* histories run: nperbatch * nbatch */
 } omc_dev_result;
 
-/* Queues a request; it is admitted once its dose grid and particle share fit
- * the HBM budget.  Returns a ticket >= 0. */
+/* Queues a request (its source is copied); it is admitted once its dose
+ * grid fits the HBM budget next to the resident tables, phantoms and
+ * particle queues.  Histories are nhist / nbatch per batch (at least 1).
+ * Returns a ticket >= 0. */
 int  omc_dev_submit(omc_dev_ctx *ctx, const omc_dev_request *req);
-/* Advances the engine by one transport generation (admits waiting requests
- * whose memory fits, launches source + photon + electron kernels for all
- * in-flight requests).  Returns the number of requests still unfinished. */
+/* Advances the engine: admits waiting requests whose memory fits, starts a
+ * transport generation on every idle pipeline that has particles or
+ * admitted histories (new primaries of all admitted requests share each
+ * generation), waits for one generation to end and finishes the requests
+ * whose last particle is gone.  Returns the number of submitted requests not
+ * yet finished (0 when idle), or a negative error. */
 int  omc_dev_step(omc_dev_ctx *ctx);
-/* Result of a ticket; when done, sum/sum2 ([nreg] each, may be NULL) get the
- * request's batch sums and the request's device memory is released. */
+/* Result of a ticket.  When done, sum/sum2 ([nreg] each, may be NULL) get
+ * the request's batch sums (set, not added) and the ticket is released. */
 int  omc_dev_poll(omc_dev_ctx *ctx, int ticket, omc_dev_result *res,
                   double *sum, double *sum2);
 

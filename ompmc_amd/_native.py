@@ -9,6 +9,8 @@ import ctypes as C
 import os
 from pathlib import Path
 
+import numpy as np
+
 PKG = Path(__file__).resolve().parent
 LIBDIR = PKG / "lib"
 DATA = PKG / "data"
@@ -109,6 +111,26 @@ class SourceBeamlets(C.Structure):
     _fields_ = [("spec", Spectrum), ("charge", C.c_int), ("nbeamlets", C.c_int),
                 ("nbeams", C.c_int), ("ibeam", _ip), ("src", _dp), ("corner", _dp),
                 ("side1", _dp), ("side2", _dp)]
+
+
+def beamlet_source(spec, ibeam, src, corner, side1, side2, charge=0):
+    """omc_source_beamlets over numpy arrays; the arrays are kept alive on
+    the returned structure."""
+    arrs = dict(ibeam=np.ascontiguousarray(ibeam, dtype=np.int32),
+                src=np.ascontiguousarray(src, dtype=np.float64).reshape(-1),
+                corner=np.ascontiguousarray(corner, dtype=np.float64).reshape(-1),
+                side1=np.ascontiguousarray(side1, dtype=np.float64).reshape(-1),
+                side2=np.ascontiguousarray(side2, dtype=np.float64).reshape(-1))
+    s = SourceBeamlets()
+    s.spec = spec
+    s.charge = charge
+    s.nbeamlets = len(arrs["ibeam"])
+    s.nbeams = len(arrs["src"]) // 3
+    s.ibeam = arrs["ibeam"].ctypes.data_as(_ip)
+    for k in ("src", "corner", "side1", "side2"):
+        setattr(s, k, arrs[k].ctypes.data_as(_dp))
+    s._keep = arrs
+    return s
 
 
 _host = None

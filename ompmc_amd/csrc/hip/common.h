@@ -189,6 +189,13 @@ struct DevJob {
     double *ledger;          // optional [3*nhist]
 };
 
+// A run of consecutive histories [first, first + n) of one job, placed at
+// thread offset `off` of a source launch.
+struct SourceSeg {
+    uint64_t first;
+    uint32_t job, n, off, pad;
+};
+
 // Structure-of-arrays particle queue.
 template <typename R> struct Queue {
     R *e, *x, *y, *z, *u, *v, *w, *wt;
@@ -236,7 +243,9 @@ __device__ __forceinline__ void q_store(const Queue<R> &q, uint32_t i, const Par
 struct DevStatus {
     uint32_t overflow;       // a push found its queue full
     uint32_t killed;         // particles stopped by the per-particle step limit
-    unsigned long long photon_steps, electron_steps, deposits;
+    unsigned long long photon_steps, electron_steps;   // transport loop trips
+    unsigned long long photon_cross, electron_cross;   // voxel boundaries crossed
+    unsigned long long photon_dep, electron_dep;       // ausgab calls
 };
 
 }  // namespace omc

@@ -1,10 +1,19 @@
 // Host side of the device engine: the C ABI declared in include/omc_dev.h.
 //
-// A context owns one HIP stream, the uploaded tables, resident phantoms and
-// four particle queues (photons and charged particles, current and next
-// generation).  A run is processed in chunks of primaries; each chunk is
-// sampled by the source kernel and then transported generation by
-// generation until both queues are empty.
+// A context owns the uploaded tables, resident phantoms and a few transport
+// pipelines.  A pipeline is one HIP stream with four particle queues
+// (photons and charged particles, current and next generation); it runs one
+// generation at a time -- source (new primaries), photon kernel, charged
+// particle kernel -- and the pipelines run concurrently, so the tail of one
+// generation (its slowest particle) overlaps the bulk of the others.
+//
+// Work arrives as jobs: one dosxyz field, one beamlet or one serving
+// request, each with its own FP64 tally [nbatch][nreg] in HBM.  Jobs are
+// admitted in order when their memory fits the HBM budget; their histories
+// are handed to the pipelines as they drain (round robin over the admitted
+// jobs), and a job is finished once every history has been injected and no
+// pipeline holds one of its particles.  Every particle carries its own
+// Philox stream, so the schedule never changes a history.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -15,6 +24,7 @@
 #include <cstring>
 #include <deque>
 #include <map>
+#include <thread>
 #include <vector>
 
 #include "omc_dev.h"
@@ -52,12 +62,94 @@ struct DevBuf {
 
 constexpr int MAX_PHANTOMS = 16;
 constexpr int MAX_JOBS = 4096;
+constexpr int MAX_SEGS = 512;
+constexpr int MAX_PIPES = 8;
+constexpr int DEFAULT_PIPES = 4;
+constexpr int64_t DEFAULT_CHUNK = 1 << 18;
 
 struct PhantomSlot {
     bool used = false;
     int nreg = 0;
-    int nmed = 0;
-    DevBuf xb, yb, zb, med, rhof;
+    int isize = 0, jsize = 0, ksize = 0;
+    DevBuf xb, yb, zb, med, rhof, rho;
+};
+
+// Host mirror of a pipeline's small device state, in pinned memory so the
+// end-of-generation copies are asynchronous.
+struct PipeMirror {
+    uint32_t counts[8];
+    DevStatus status;
+    uint32_t flags[MAX_JOBS];
+    SourceSeg segs[MAX_SEGS];
+};
+
+struct Pipe {
+    hipStream_t stream = nullptr;
+    void *q[4][13] = {};          // [photon cur/next, charged cur/next][field]
+    uint32_t *d_counts = nullptr; // 4 tails + 2 fetch counters
+    DevStatus *d_status = nullptr;
+    uint32_t *d_flags = nullptr;  // [MAX_JOBS]
+    SourceSeg *d_segs = nullptr;  // [MAX_SEGS]
+    PipeMirror *h = nullptr;
+    hipEvent_t ev_src0 = nullptr, ev_src1 = nullptr, ev_ph0 = nullptr, ev_ph1 = nullptr;
+    hipEvent_t ev_el0 = nullptr, ev_el1 = nullptr, ev_done = nullptr;
+    int cur = 0;
+    uint32_t nP = 0, nE = 0;      // particles waiting in the current queues
+    bool inflight = false;
+    bool had_src = false, had_ph = false, had_el = false;
+    uint32_t injected = 0;        // primaries injected by the generation in flight
+    std::vector<int> open;        // jobs that may hold particles here
+    DevStatus base;               // status at the start of the current run
+};
+
+enum WorkState { W_WAITING = 0, W_RUNNING = 1, W_DONE = 2, W_FAILED = 3 };
+
+struct RunSink;
+
+// An admission unit: a source plus its histories.  Serving requests and the
+// beamlets / field of a synchronous run are all Works.
+struct Work {
+    int ticket = -1;              // serving ticket, or -1 for a synchronous run
+    RunSink *sink = nullptr;      // synchronous run receiving the result
+    int index = 0;                // beamlet index within the run
+    int phantom = 0;
+    DevJob tmpl;                  // source and stream fields; memory filled on admission
+    std::vector<double> cdf;      // [2*OMC_INVDIM] inverse CDF
+    int64_t nhist = 0, nperbatch = 0;
+    int nbatch = 1;
+    int64_t nreg = 0;
+    bool ledger = false;
+    int state = W_WAITING;
+    int status = OMC_OK;
+    int job = -1;
+    double submit_ms = 0, admit_ms = 0, finish_ms = 0;
+    std::vector<double> sum, sum2;
+};
+
+// Where a synchronous run wants its results.
+struct RunSink {
+    bool beamlets = false;
+    double *sum = nullptr, *sum2 = nullptr;   // [nreg] or [nb*nreg]
+    double *ledger = nullptr;                 // dosxyz only
+    bool sparse = false;
+    double rel_thr = 0.0;
+    std::vector<std::vector<int64_t>> rows;
+    std::vector<std::vector<double>> vals;
+    int remaining = 0;
+    int rc = OMC_OK;
+};
+
+struct JobRec {
+    bool used = false;
+    Work *w = nullptr;
+    int64_t injected = 0;
+    uint32_t pipes = 0;           // pipelines that may hold its particles
+    int64_t quantum = 0;          // histories per turn in the round robin
+    double *tally = nullptr;
+    size_t tally_bytes = 0;
+    double *cdf = nullptr;
+    double *ledger = nullptr;
+    size_t ledger_bytes = 0;
 };
 
 }  // namespace
@@ -65,14 +157,14 @@ struct PhantomSlot {
 struct omc_dev_ctx {
     int device = 0;
     int precision = OMC_PREC_FP64;
-    hipStream_t stream = nullptr;
+    hipStream_t aux = nullptr;    // admission and finishing work
     int64_t budget = 0;
     int64_t used = 0;
     int64_t chunk = 0;
     uint32_t qcap = 0;
     int cus = 256;
+    std::chrono::steady_clock::time_point t0;
 
-    std::vector<void *> allocs;
     // tables
     bool media = false;
     int nmed = 0;
@@ -81,22 +173,36 @@ struct omc_dev_ctx {
     // phantoms
     PhantomSlot ph[MAX_PHANTOMS];
     void *d_phantoms = nullptr;       // DevPhantom<R>[MAX_PHANTOMS]
-    std::vector<unsigned char> h_phantoms;
+    // finishing scratch (fold output + sparse column), sized for the largest phantom
+    int64_t fin_nvox = -1;
+    double *fin_sums = nullptr;       // [2*nreg]
+    SparseScratch sp = {};
+    size_t fin_bytes = 0;
     // jobs
     DevJob *d_jobs = nullptr;
-    // queues: [0] photon cur, [1] photon next, [2] electron cur, [3] electron next
-    void *q[4][13] = {};
-    uint32_t *d_counts = nullptr;     // 4 queue tails + 2 fetch counters
-    DevStatus *d_status = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-
+    JobRec jobs[MAX_JOBS];
+    int jobs_hi = 0;                  // one past the highest used slot
+    std::deque<int> runnable;         // admitted jobs with histories left to inject
+    std::deque<Work *> waiting;       // works not yet admitted
+    std::multimap<size_t, void *> pool;   // freed tallies kept for reuse
+    // pipelines
+    std::vector<Pipe> pipes;
+    omc_dev_stats acc;                // accumulated over the current run
     // serving
-    struct Req;
-    std::map<int, Req *> reqs;
+    std::map<int, Work *> tickets;
     int next_ticket = 0;
+    int unfinished = 0;               // submitted tickets not yet done
+    // last sparse Dij that did not fit the caller's arrays
+    std::vector<std::vector<int64_t>> kept_rows;
+    std::vector<std::vector<double>> kept_vals;
 };
 
 namespace {
+
+double now_ms(const omc_dev_ctx *c)
+{
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - c->t0).count();
+}
 
 int dalloc(omc_dev_ctx *c, void **p, size_t bytes)
 {
@@ -116,30 +222,77 @@ void dfree(omc_dev_ctx *c, void *p, size_t bytes)
     c->used -= (int64_t)bytes;
 }
 
+// Tallies come and go with jobs; buffers of a size seen before are reused
+// instead of returned, and returned only when the budget needs the room.
+int pool_alloc(omc_dev_ctx *c, void **p, size_t bytes)
+{
+    auto it = c->pool.find(bytes);
+    if (it != c->pool.end()) {
+        *p = it->second;
+        c->pool.erase(it);
+        return OMC_OK;
+    }
+    while (c->budget > 0 && c->used + (int64_t)bytes > c->budget && !c->pool.empty()) {
+        auto big = std::prev(c->pool.end());
+        dfree(c, big->second, big->first);
+        c->pool.erase(big);
+    }
+    return dalloc(c, p, bytes);
+}
+
+void pool_free(omc_dev_ctx *c, void *p, size_t bytes)
+{
+    if (p) c->pool.emplace(bytes, p);
+}
+
 template <typename R> size_t particle_bytes() { return 8 * sizeof(R) + 6 * sizeof(uint32_t); }
 
-template <typename R> Queue<R> queue_view(omc_dev_ctx *c, int k)
+template <typename R> Queue<R> queue_view(const omc_dev_ctx *c, const Pipe &P, int k)
 {
     Queue<R> q;
-    void **a = c->q[k];
+    void *const *a = P.q[k];
     q.e = (R *)a[0]; q.x = (R *)a[1]; q.y = (R *)a[2]; q.z = (R *)a[3];
     q.u = (R *)a[4]; q.v = (R *)a[5]; q.w = (R *)a[6]; q.wt = (R *)a[7];
     q.ir = (int32_t *)a[8]; q.meta = (uint32_t *)a[9]; q.hist = (uint32_t *)a[10];
     q.k0 = (uint32_t *)a[11]; q.k1 = (uint32_t *)a[12];
     q.ctr = (uint32_t *)a[12] + c->qcap;     // k1 and ctr share one allocation
-    q.count = c->d_counts + k;
+    q.count = P.d_counts + k;
     q.cap = c->qcap;
     return q;
 }
 
-template <typename R> int alloc_queues(omc_dev_ctx *c)
+template <typename R> int alloc_pipe(omc_dev_ctx *c, Pipe &P)
 {
+    if (hipStreamCreateWithFlags(&P.stream, hipStreamNonBlocking) != hipSuccess) return fail(OMC_EDEVICE, "stream");
+    hipEvent_t *evs[] = { &P.ev_src0, &P.ev_src1, &P.ev_ph0, &P.ev_ph1, &P.ev_el0, &P.ev_el1, &P.ev_done };
+    for (auto *e : evs) HIPCHK(hipEventCreate(e));
     for (int k = 0; k < 4; ++k) {
-        for (int f = 0; f < 8; ++f) OMC_TRY_DEV(dalloc(c, &c->q[k][f], sizeof(R) * (size_t)c->qcap));
-        for (int f = 8; f < 12; ++f) OMC_TRY_DEV(dalloc(c, &c->q[k][f], 4 * (size_t)c->qcap));
-        OMC_TRY_DEV(dalloc(c, &c->q[k][12], 8 * (size_t)c->qcap));
+        for (int f = 0; f < 8; ++f) OMC_TRY_DEV(dalloc(c, &P.q[k][f], sizeof(R) * (size_t)c->qcap));
+        for (int f = 8; f < 12; ++f) OMC_TRY_DEV(dalloc(c, &P.q[k][f], 4 * (size_t)c->qcap));
+        OMC_TRY_DEV(dalloc(c, &P.q[k][12], 8 * (size_t)c->qcap));
     }
+    OMC_TRY_DEV(dalloc(c, (void **)&P.d_counts, 8 * sizeof(uint32_t)));
+    OMC_TRY_DEV(dalloc(c, (void **)&P.d_status, sizeof(DevStatus)));
+    OMC_TRY_DEV(dalloc(c, (void **)&P.d_flags, MAX_JOBS * sizeof(uint32_t)));
+    OMC_TRY_DEV(dalloc(c, (void **)&P.d_segs, MAX_SEGS * sizeof(SourceSeg)));
+    HIPCHK(hipHostMalloc((void **)&P.h, sizeof(PipeMirror), hipHostMallocDefault));
+    memset(P.h, 0, sizeof(PipeMirror));
+    HIPCHK(hipMemset(P.d_counts, 0, 8 * sizeof(uint32_t)));
+    HIPCHK(hipMemset(P.d_status, 0, sizeof(DevStatus)));
+    memset(&P.base, 0, sizeof P.base);
     return OMC_OK;
+}
+
+void free_pipe(Pipe &P)
+{
+    if (P.stream) (void)hipStreamSynchronize(P.stream);
+    for (auto &qq : P.q) for (auto &p : qq) if (p) { (void)hipFree(p); p = nullptr; }
+    void *ptrs[] = { P.d_counts, P.d_status, P.d_flags, P.d_segs };
+    for (void *p : ptrs) if (p) (void)hipFree(p);
+    if (P.h) (void)hipHostFree(P.h);
+    hipEvent_t evs[] = { P.ev_src0, P.ev_src1, P.ev_ph0, P.ev_ph1, P.ev_el0, P.ev_el1, P.ev_done };
+    for (auto e : evs) if (e) (void)hipEventDestroy(e);
+    if (P.stream) (void)hipStreamDestroy(P.stream);
 }
 
 }  // namespace
@@ -239,6 +392,33 @@ template <typename R> static DevTables<R> tables_view(omc_dev_ctx *c)
 }
 
 // --------------------------------------------------------------- phantoms --
+// Scratch for finishing a job on the device: the folded sums and the sparse
+// column of a beamlet, sized for the largest resident phantom.
+static int grow_finish_scratch(omc_dev_ctx *c, int64_t nvox)
+{
+    if (nvox <= c->fin_nvox) return OMC_OK;
+    if (c->fin_sums) dfree(c, c->fin_sums, c->fin_bytes);
+    c->fin_sums = nullptr;
+    c->fin_nvox = -1;
+    const int nblk = sparse_blocks(nvox);
+    const size_t sums = 2 * sizeof(double) * (size_t)(nvox + 1);
+    const size_t dose = sizeof(double) * (size_t)nvox;
+    const size_t blk = sizeof(uint32_t) * ((size_t)nblk + 2);
+    const size_t rows = sizeof(int64_t) * (size_t)nvox, vals = sizeof(double) * (size_t)nvox;
+    auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    c->fin_bytes = up(sums) + up(dose) + up(8) + up(blk) + up(rows) + up(vals);
+    char *p = nullptr;
+    OMC_TRY_DEV(dalloc(c, (void **)&p, c->fin_bytes));
+    c->fin_sums = (double *)p;              p += up(sums);
+    c->sp.dose = (double *)p;               p += up(dose);
+    c->sp.maxkey = (unsigned long long *)p; p += up(8);
+    c->sp.blk = (uint32_t *)p;              p += up(blk);
+    c->sp.rows = (int64_t *)p;              p += up(rows);
+    c->sp.vals = (double *)p;
+    c->fin_nvox = nvox;
+    return OMC_OK;
+}
+
 template <typename R> static int upload_phantom(omc_dev_ctx *c, const omc_phantom *ph, const omc_regions *reg, int *id)
 {
     int slot = -1;
@@ -246,7 +426,8 @@ template <typename R> static int upload_phantom(omc_dev_ctx *c, const omc_phanto
     if (slot < 0) return fail(OMC_ERANGE, "more than %d resident phantoms", MAX_PHANTOMS);
     if (!c->media) return fail(OMC_EARG, "upload media before phantoms");
     const int nreg = reg->nreg;
-    if ((int64_t)nreg != (int64_t)ph->isize * ph->jsize * ph->ksize + 1)
+    const int64_t nvox = (int64_t)ph->isize * ph->jsize * ph->ksize;
+    if ((int64_t)nreg != nvox + 1)
         return fail(OMC_EARG, "regions (%d) do not match phantom %dx%dx%d", nreg, ph->isize, ph->jsize, ph->ksize);
     PhantomSlot &P = c->ph[slot];
     std::vector<uint8_t> med(nreg);
@@ -267,15 +448,18 @@ template <typename R> static int upload_phantom(omc_dev_ctx *c, const omc_phanto
         }
     }
     for (int m = 1; m <= MAXMED; ++m) { if (ecut[m] < 0) ecut[m] = 0.0; if (pcut[m] < 0) pcut[m] = 0.0; }
+    OMC_TRY_DEV(grow_finish_scratch(c, nvox));
     P.nreg = nreg;
+    P.isize = ph->isize; P.jsize = ph->jsize; P.ksize = ph->ksize;
     P.xb.bytes = sizeof(double) * (ph->isize + 1);
     P.yb.bytes = sizeof(double) * (ph->jsize + 1);
     P.zb.bytes = sizeof(double) * (ph->ksize + 1);
     P.med.bytes = nreg;
     P.rhof.bytes = sizeof(R) * (size_t)nreg;
-    DevBuf *bufs[] = { &P.xb, &P.yb, &P.zb, &P.med, &P.rhof };
-    const void *src[] = { ph->xbounds, ph->ybounds, ph->zbounds, med.data(), rhof.data() };
-    for (int b = 0; b < 5; ++b) {
+    P.rho.bytes = sizeof(double) * (size_t)nvox;
+    DevBuf *bufs[] = { &P.xb, &P.yb, &P.zb, &P.med, &P.rhof, &P.rho };
+    const void *src[] = { ph->xbounds, ph->ybounds, ph->zbounds, med.data(), rhof.data(), ph->rho };
+    for (int b = 0; b < 6; ++b) {
         OMC_TRY_DEV(dalloc(c, &bufs[b]->p, bufs[b]->bytes));
         HIPCHK(hipMemcpy(bufs[b]->p, src[b], bufs[b]->bytes, hipMemcpyHostToDevice));
     }
@@ -291,145 +475,429 @@ template <typename R> static int upload_phantom(omc_dev_ctx *c, const omc_phanto
     return OMC_OK;
 }
 
-// ---------------------------------------------------------- transport -----
-struct RunJob {
-    DevJob j;
-    int64_t nhist;
-};
-
-template <typename R>
-static int transport_chunk(omc_dev_ctx *c, const std::vector<std::pair<int, std::pair<uint64_t, uint32_t>>> &src,
-                           omc_dev_stats *stats)
+// -------------------------------------------------------------- scheduler --
+static void release_job(omc_dev_ctx *c, int j)
 {
-    // src: (job id, (first history, count)) segments sampled into the queues
-    hipStream_t s = c->stream;
-    launch_set_u32(c->d_counts, 8, 0u, s);
-    Queue<R> qp = queue_view<R>(c, 0), qe = queue_view<R>(c, 2);
-    auto t0 = std::chrono::steady_clock::now();
-    for (auto &seg : src)
-        Launch<R>::source(c->d_jobs, (const DevPhantom<R> *)c->d_phantoms, seg.first, seg.second.first,
-                          seg.second.second, qp, qe, c->d_status, s);
-    int cur = 0;
-    uint32_t counts[4];
-    HIPCHK(hipMemcpyAsync(counts, c->d_counts, sizeof counts, hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
-    if (stats) stats->source_kernel_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    {
-        DevStatus st0;
-        HIPCHK(hipMemcpy(&st0, c->d_status, sizeof st0, hipMemcpyDeviceToHost));
-        if (st0.overflow) return fail(OMC_EOVERFLOW, "source overflow (capacity %u)", c->qcap);
+    JobRec &J = c->jobs[j];
+    pool_free(c, J.tally, J.tally_bytes);
+    if (J.cdf) dfree(c, J.cdf, 2 * OMC_INVDIM * sizeof(double));
+    if (J.ledger) dfree(c, J.ledger, J.ledger_bytes);
+    J = JobRec();
+    while (c->jobs_hi > 0 && !c->jobs[c->jobs_hi - 1].used) c->jobs_hi--;
+}
+
+static void end_work(omc_dev_ctx *c, Work *w, int status)
+{
+    w->state = status == OMC_OK ? W_DONE : W_FAILED;
+    w->status = status;
+    w->finish_ms = now_ms(c);
+    if (w->sink) {
+        if (status != OMC_OK && w->sink->rc == OMC_OK) w->sink->rc = status;
+        w->sink->remaining--;
+    } else if (w->ticket >= 0) {
+        c->unfinished--;
     }
-    uint32_t nP = std::min(counts[0], c->qcap), nE = std::min(counts[2], c->qcap);
-    const int max_grid = c->cus * 8;
-    while (nP || nE) {
-        int nxt = cur ^ 1;
-        // reset next-generation tails and the fetch counters
-        launch_set_u32(c->d_counts + nxt, 1, 0u, s);
-        launch_set_u32(c->d_counts + 2 + nxt, 1, 0u, s);
-        launch_set_u32(c->d_counts + 4, 2, 0u, s);
-        KArgs<R> A;
-        A.T = tables_view<R>(c);
-        A.phantoms = (const DevPhantom<R> *)c->d_phantoms;
-        A.jobs = c->d_jobs;
-        A.qp = queue_view<R>(c, nxt);
-        A.qe = queue_view<R>(c, 2 + nxt);
-        A.status = c->d_status;
-        float ms_p = 0, ms_e = 0;
-        if (nP) {
-            A.qin = queue_view<R>(c, cur);
-            A.in_begin = 0;
-            A.in_end = nP;
-            A.fetch = c->d_counts + 4;
-            int grid = std::min<int64_t>(max_grid, ((int64_t)nP + 255) / 256);
-            HIPCHK(hipEventRecord(c->ev0, s));
-            Launch<R>::photon(A, grid, s);
-            HIPCHK(hipEventRecord(c->ev1, s));
-            HIPCHK(hipEventSynchronize(c->ev1));
-            HIPCHK(hipEventElapsedTime(&ms_p, c->ev0, c->ev1));
-            if (stats) { stats->photon_kernel_ms += ms_p; stats->photon_launches++; }
+}
+
+// Bytes a work needs on admission.
+static size_t work_bytes(const Work *w)
+{
+    return sizeof(double) * (size_t)w->nreg * w->nbatch + 2 * OMC_INVDIM * sizeof(double) +
+           (w->ledger ? sizeof(double) * 3 * (size_t)w->nhist : 0);
+}
+
+static int any_running(const omc_dev_ctx *c)
+{
+    for (int j = 0; j < c->jobs_hi; ++j) if (c->jobs[j].used) return 1;
+    return 0;
+}
+
+// Admit waiting works in order while their memory fits.  A work that cannot
+// fit even with nothing else admitted fails with OMC_EBUDGET.
+static int admit(omc_dev_ctx *c)
+{
+    while (!c->waiting.empty()) {
+        Work *w = c->waiting.front();
+        int slot = -1;
+        for (int j = 0; j < MAX_JOBS; ++j) if (!c->jobs[j].used) { slot = j; break; }
+        if (slot < 0) return OMC_OK;
+        int64_t pooled = 0;
+        for (auto &kv : c->pool) pooled += (int64_t)kv.first;
+        if (c->budget > 0 && c->used - pooled + (int64_t)work_bytes(w) > c->budget) {
+            if (any_running(c)) return OMC_OK;          // wait for memory to free up
+            c->waiting.pop_front();
+            fail(OMC_EBUDGET, "request needs %zu bytes, budget leaves %lld", work_bytes(w),
+                 (long long)(c->budget - (c->used - pooled)));
+            end_work(c, w, OMC_EBUDGET);
+            continue;
         }
-        if (nE) {
-            A.qin = queue_view<R>(c, 2 + cur);
-            A.in_begin = 0;
-            A.in_end = nE;
-            A.fetch = c->d_counts + 5;
-            int grid = std::min<int64_t>(max_grid, ((int64_t)nE + 255) / 256);
-            HIPCHK(hipEventRecord(c->ev0, s));
-            Launch<R>::electron(A, grid, s);
-            HIPCHK(hipEventRecord(c->ev1, s));
-            HIPCHK(hipEventSynchronize(c->ev1));
-            HIPCHK(hipEventElapsedTime(&ms_e, c->ev0, c->ev1));
-            if (stats) { stats->electron_kernel_ms += ms_e; stats->electron_launches++; }
+        JobRec &J = c->jobs[slot];
+        J = JobRec();
+        J.tally_bytes = sizeof(double) * (size_t)w->nreg * w->nbatch;
+        int rc = pool_alloc(c, (void **)&J.tally, J.tally_bytes);
+        if (!rc) rc = dalloc(c, (void **)&J.cdf, 2 * OMC_INVDIM * sizeof(double));
+        if (!rc && w->ledger) {
+            J.ledger_bytes = sizeof(double) * 3 * (size_t)w->nhist;
+            rc = dalloc(c, (void **)&J.ledger, J.ledger_bytes);
         }
-        HIPCHK(hipGetLastError());
-        HIPCHK(hipMemcpyAsync(counts, c->d_counts, sizeof counts, hipMemcpyDeviceToHost, s));
-        DevStatus st;
-        HIPCHK(hipMemcpyAsync(&st, c->d_status, sizeof st, hipMemcpyDeviceToHost, s));
-        HIPCHK(hipStreamSynchronize(s));
-        if (st.overflow) return fail(OMC_EOVERFLOW, "particle queue overflow (capacity %u)", c->qcap);
-        if (stats) {
-            stats->generations++;
-            stats->transport_ms += ms_p + ms_e;
-            stats->photons_pushed += counts[nxt];
-            stats->electrons_pushed += counts[2 + nxt];
+        if (rc) {
+            J.used = true;
+            release_job(c, slot);
+            if (any_running(c)) return OMC_OK;
+            c->waiting.pop_front();
+            end_work(c, w, rc);
+            continue;
         }
-        nP = std::min(counts[nxt], c->qcap);
-        nE = std::min(counts[2 + nxt], c->qcap);
-        cur = nxt;
+        J.used = true;
+        J.w = w;
+        J.quantum = w->ticket >= 0 ? std::max<int64_t>(4096, c->chunk / 8) : INT64_MAX;
+        launch_zero_f64(J.tally, w->nreg * w->nbatch, c->aux);
+        if (J.ledger) launch_zero_f64(J.ledger, 3 * w->nhist, c->aux);
+        HIPCHK(hipMemcpyAsync(J.cdf, w->cdf.data(), 2 * OMC_INVDIM * sizeof(double), hipMemcpyHostToDevice, c->aux));
+        DevJob d = w->tmpl;
+        d.phantom = w->phantom;
+        d.cdf1 = J.cdf;
+        d.cdf2 = J.cdf + OMC_INVDIM;
+        d.nperbatch = w->nperbatch;
+        d.tally = J.tally;
+        d.ledger = J.ledger;
+        HIPCHK(hipMemcpyAsync(c->d_jobs + slot, &d, sizeof d, hipMemcpyHostToDevice, c->aux));
+        HIPCHK(hipStreamSynchronize(c->aux));
+        c->jobs_hi = std::max(c->jobs_hi, slot + 1);
+        w->job = slot;
+        w->state = W_RUNNING;
+        w->admit_ms = now_ms(c);
+        c->waiting.pop_front();
+        c->runnable.push_back(slot);
     }
     return OMC_OK;
 }
 
-static void fill_source_job(DevJob &j, const omc_spectrum &spec, int charge, double *cdf_dev)
+// The job's last particle is gone: fold its batches (accumEndep,
+// omc_dosxyz.c:696-717) and deliver the result.
+static int finish_job(omc_dev_ctx *c, int j)
 {
-    j.charge = charge;
-    j.spectrum = spec.spectrum;
-    j.energy = spec.energy;
-    j.deltak = spec.deltak;
-    j.cdf1 = cdf_dev;
-    j.cdf2 = cdf_dev + OMC_INVDIM;
+    JobRec &J = c->jobs[j];
+    Work *w = J.w;
+    const int64_t nreg = w->nreg;
+    const PhantomSlot &P = c->ph[w->phantom];
+    hipStream_t s = c->aux;
+    int rc = OMC_OK;
+    RunSink *k = w->sink;
+    const bool want_sums = !k || k->sum || !k->beamlets;
+    if (want_sums) {
+        launch_fold(J.tally, w->nbatch, nreg, c->fin_sums, c->fin_sums + nreg, s);
+        std::vector<double> h(2 * (size_t)nreg);
+        if (hipMemcpyAsync(h.data(), c->fin_sums, h.size() * sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess)
+            rc = fail(OMC_EDEVICE, "copy tallies");
+        if (!rc && !k) {
+            w->sum.assign(h.begin(), h.begin() + nreg);
+            w->sum2.assign(h.begin() + nreg, h.end());
+        } else if (!rc && k->sum) {
+            double *s1 = k->sum + (k->beamlets ? (size_t)w->index * nreg : 0);
+            double *s2 = k->sum2 + (k->beamlets ? (size_t)w->index * nreg : 0);
+            for (int64_t i = 0; i < nreg; ++i) { s1[i] += h[i]; s2[i] += h[nreg + i]; }
+        }
+    }
+    if (!rc && k && k->ledger && J.ledger &&
+        hipMemcpy(k->ledger, J.ledger, J.ledger_bytes, hipMemcpyDeviceToHost) != hipSuccess)
+        rc = fail(OMC_EDEVICE, "copy ledger");
+    if (!rc && k && k->sparse) {
+        const double inc = (double)(w->nperbatch * (int64_t)w->nbatch);
+        launch_beamlet_column(J.tally, w->nbatch, P.isize, P.jsize, P.ksize, (const double *)P.xb.p,
+                              (const double *)P.yb.p, (const double *)P.zb.p, (const double *)P.rho.p, inc,
+                              k->rel_thr, c->sp, s);
+        const int nblk = sparse_blocks((int64_t)P.isize * P.jsize * P.ksize);
+        uint32_t nnz = 0;
+        if (hipMemcpyAsync(&nnz, c->sp.blk + nblk, sizeof nnz, hipMemcpyDeviceToHost, s) != hipSuccess ||
+            hipStreamSynchronize(s) != hipSuccess)
+            rc = fail(OMC_EDEVICE, "sparse count");
+        if (!rc) {
+            auto &rows = k->rows[w->index];
+            auto &vals = k->vals[w->index];
+            rows.resize(nnz);
+            vals.resize(nnz);
+            if (nnz && (hipMemcpy(rows.data(), c->sp.rows, nnz * sizeof(int64_t), hipMemcpyDeviceToHost) != hipSuccess ||
+                        hipMemcpy(vals.data(), c->sp.vals, nnz * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess))
+                rc = fail(OMC_EDEVICE, "copy sparse column");
+        }
+    }
+    release_job(c, j);
+    end_work(c, w, rc);
+    return rc;
 }
 
-template <typename R>
-static int run_jobs(omc_dev_ctx *c, std::vector<RunJob> &jobs, omc_dev_stats *stats)
+// Hand up to `room` new primaries to pipeline p (round robin over the
+// admitted jobs); fills the pinned segment table.
+static int pull_segments(omc_dev_ctx *c, int pi, int64_t room, uint32_t &injP, uint32_t &injE)
 {
-    if ((int)jobs.size() > MAX_JOBS) return fail(OMC_ERANGE, "more than %d jobs in one run", MAX_JOBS);
-    std::vector<DevJob> dj(jobs.size());
-    for (size_t i = 0; i < jobs.size(); ++i) dj[i] = jobs[i].j;
-    HIPCHK(hipMemcpy(c->d_jobs, dj.data(), dj.size() * sizeof(DevJob), hipMemcpyHostToDevice));
-    HIPCHK(hipMemset(c->d_status, 0, sizeof(DevStatus)));
-    // chunks: consecutive history ranges over the jobs
-    std::vector<std::pair<int, std::pair<uint64_t, uint32_t>>> segs;
-    int64_t in_chunk = 0;
-    for (size_t i = 0; i < jobs.size(); ++i) {
-        int64_t done = 0;
-        while (done < jobs[i].nhist) {
-            int64_t take = std::min<int64_t>(jobs[i].nhist - done, c->chunk - in_chunk);
-            segs.push_back({ (int)i, { (uint64_t)done, (uint32_t)take } });
-            done += take;
-            in_chunk += take;
-            if (in_chunk == c->chunk) {
-                OMC_TRY_DEV(transport_chunk<R>(c, segs, stats));
-                if (stats) stats->chunks++;
-                segs.clear();
-                in_chunk = 0;
+    Pipe &P = c->pipes[pi];
+    int nseg = 0;
+    uint32_t off = 0;
+    injP = injE = 0;
+    size_t turns = c->runnable.size();
+    while (room > 0 && nseg < MAX_SEGS && turns-- > 0 && !c->runnable.empty()) {
+        int j = c->runnable.front();
+        c->runnable.pop_front();
+        JobRec &J = c->jobs[j];
+        int64_t left = J.w->nhist - J.injected;
+        int64_t take = std::min<int64_t>(std::min<int64_t>(left, room), J.quantum);
+        SourceSeg &sg = P.h->segs[nseg++];
+        sg.first = (uint64_t)J.injected;
+        sg.job = (uint32_t)j;
+        sg.n = (uint32_t)take;
+        sg.off = off;
+        sg.pad = 0;
+        off += (uint32_t)take;
+        room -= take;
+        J.injected += take;
+        if (J.w->tmpl.charge == 0) injP += (uint32_t)take; else injE += (uint32_t)take;
+        if (!(J.pipes & (1u << pi))) {
+            J.pipes |= 1u << pi;
+            P.open.push_back(j);
+        }
+        if (J.injected < J.w->nhist) c->runnable.push_back(j);
+    }
+    return nseg;
+}
+
+template <typename R> static int launch_generation(omc_dev_ctx *c, int pi)
+{
+    Pipe &P = c->pipes[pi];
+    hipStream_t s = P.stream;
+    const int nxt = P.cur ^ 1;
+    const int64_t live = (int64_t)P.nP + P.nE;
+    uint32_t injP = 0, injE = 0;
+    const int nseg = live < c->chunk ? pull_segments(c, pi, c->chunk - live, injP, injE) : 0;
+    const uint32_t ntot = injP + injE;
+    const uint32_t inP = P.nP + injP, inE = P.nE + injE;
+    if (!inP && !inE) return OMC_OK;
+    launch_gen_reset(P.d_counts, nxt, s);
+    const Queue<R> qpc = queue_view<R>(c, P, P.cur), qec = queue_view<R>(c, P, 2 + P.cur);
+    P.had_src = nseg > 0;
+    if (P.had_src) {
+        HIPCHK(hipMemcpyAsync(P.d_segs, P.h->segs, nseg * sizeof(SourceSeg), hipMemcpyHostToDevice, s));
+        HIPCHK(hipEventRecord(P.ev_src0, s));
+        Launch<R>::source(c->d_jobs, (const DevPhantom<R> *)c->d_phantoms, P.d_segs, nseg, ntot, qpc, qec,
+                          P.d_status, s);
+        HIPCHK(hipEventRecord(P.ev_src1, s));
+    }
+    KArgs<R> A;
+    A.T = tables_view<R>(c);
+    A.phantoms = (const DevPhantom<R> *)c->d_phantoms;
+    A.jobs = c->d_jobs;
+    A.qp = queue_view<R>(c, P, nxt);
+    A.qe = queue_view<R>(c, P, 2 + nxt);
+    A.status = P.d_status;
+    const int max_grid = c->cus * 8;
+    P.had_ph = inP > 0;
+    if (P.had_ph) {
+        A.qin = qpc;
+        A.in_count = P.d_counts + P.cur;
+        A.fetch = P.d_counts + 4;
+        HIPCHK(hipEventRecord(P.ev_ph0, s));
+        Launch<R>::photon(A, (int)std::min<int64_t>(max_grid, ((int64_t)inP + 255) / 256), s);
+        HIPCHK(hipEventRecord(P.ev_ph1, s));
+    }
+    P.had_el = inE > 0;
+    if (P.had_el) {
+        A.qin = qec;
+        A.in_count = P.d_counts + 2 + P.cur;
+        A.fetch = P.d_counts + 5;
+        HIPCHK(hipEventRecord(P.ev_el0, s));
+        Launch<R>::electron(A, (int)std::min<int64_t>(max_grid, ((int64_t)inE + 255) / 256), s);
+        HIPCHK(hipEventRecord(P.ev_el1, s));
+    }
+    HIPCHK(hipGetLastError());
+    const int njobs = std::max(c->jobs_hi, 1);
+    launch_set_u32(P.d_flags, njobs, 0u, s);
+    const Queue<R> qpn = queue_view<R>(c, P, nxt), qen = queue_view<R>(c, P, 2 + nxt);
+    launch_job_flags(qpn.meta, qpn.count, qen.meta, qen.count, c->qcap, P.d_flags, njobs, s);
+    HIPCHK(hipMemcpyAsync(P.h->counts, P.d_counts, 8 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(&P.h->status, P.d_status, sizeof(DevStatus), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(P.h->flags, P.d_flags, njobs * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipEventRecord(P.ev_done, s));
+    P.injected = ntot;
+    P.inflight = true;
+    return OMC_OK;
+}
+
+static int complete_generation(omc_dev_ctx *c, int pi)
+{
+    Pipe &P = c->pipes[pi];
+    P.inflight = false;
+    P.cur ^= 1;
+    P.nP = std::min(P.h->counts[P.cur], c->qcap);
+    P.nE = std::min(P.h->counts[2 + P.cur], c->qcap);
+    omc_dev_stats &S = c->acc;
+    float ms = 0.f;
+    if (P.had_src) {
+        HIPCHK(hipEventElapsedTime(&ms, P.ev_src0, P.ev_src1));
+        S.source_kernel_ms += ms;
+        S.injections++;
+    }
+    if (P.had_ph) {
+        HIPCHK(hipEventElapsedTime(&ms, P.ev_ph0, P.ev_ph1));
+        S.photon_kernel_ms += ms;
+        S.photon_launches++;
+    }
+    if (P.had_el) {
+        HIPCHK(hipEventElapsedTime(&ms, P.ev_el0, P.ev_el1));
+        S.electron_kernel_ms += ms;
+        S.electron_launches++;
+    }
+    S.generations++;
+    S.photons_pushed += P.h->counts[P.cur];
+    S.electrons_pushed += P.h->counts[2 + P.cur];
+    if (P.h->status.overflow)
+        return fail(OMC_EOVERFLOW, "particle queue overflow (capacity %u per queue)", c->qcap);
+    // jobs with no particle left in this pipeline
+    std::vector<int> still;
+    std::vector<int> check;
+    for (int j : P.open) {
+        if (P.h->flags[j]) { still.push_back(j); continue; }
+        c->jobs[j].pipes &= ~(1u << pi);
+        check.push_back(j);
+    }
+    P.open.swap(still);
+    int rc = OMC_OK;
+    for (int j : check) {
+        JobRec &J = c->jobs[j];
+        if (J.used && J.pipes == 0 && J.injected == J.w->nhist) {
+            int r = finish_job(c, j);
+            if (r && !rc) rc = r;
+        }
+    }
+    return rc;
+}
+
+// Stops everything after a fatal error: waits for the pipelines, empties
+// their queues and fails every admitted or waiting work.
+static void reset_engine(omc_dev_ctx *c, int rc)
+{
+    char keep[sizeof g_err];
+    memcpy(keep, g_err, sizeof keep);
+    for (auto &P : c->pipes) {
+        (void)hipStreamSynchronize(P.stream);
+        (void)hipMemsetAsync(P.d_counts, 0, 8 * sizeof(uint32_t), P.stream);
+        (void)hipMemsetAsync(P.d_status, 0, sizeof(DevStatus), P.stream);
+        (void)hipStreamSynchronize(P.stream);
+        P.cur = 0;
+        P.nP = P.nE = 0;
+        P.inflight = false;
+        P.open.clear();
+        memset(&P.base, 0, sizeof P.base);
+        memset(&P.h->status, 0, sizeof P.h->status);
+    }
+    c->runnable.clear();
+    for (int j = 0; j < c->jobs_hi; ++j) {
+        if (!c->jobs[j].used) continue;
+        Work *w = c->jobs[j].w;
+        release_job(c, j);
+        end_work(c, w, rc);
+    }
+    while (!c->waiting.empty()) {
+        Work *w = c->waiting.front();
+        c->waiting.pop_front();
+        end_work(c, w, rc);
+    }
+    memcpy(g_err, keep, sizeof keep);
+}
+
+// One scheduler turn: admit, start a generation on every idle pipeline that
+// has work, wait for one generation to end and process it.  Returns 1 if a
+// generation was processed, 0 if the engine is idle.
+template <typename R> static int turn(omc_dev_ctx *c)
+{
+    int rc = admit(c);
+    for (size_t p = 0; !rc && p < c->pipes.size(); ++p)
+        if (!c->pipes[p].inflight) rc = launch_generation<R>(c, (int)p);
+    if (rc) { reset_engine(c, rc); return rc; }
+    bool any = false;
+    for (auto &P : c->pipes) any |= P.inflight;
+    if (!any) return 0;
+    int spins = 0;
+    for (;;) {
+        int oldest = -1;
+        for (size_t p = 0; p < c->pipes.size(); ++p) {
+            Pipe &P = c->pipes[p];
+            if (!P.inflight) continue;
+            hipError_t e = hipEventQuery(P.ev_done);
+            if (e == hipSuccess) { oldest = (int)p; break; }
+            if (e != hipErrorNotReady) {
+                rc = fail(OMC_EDEVICE, "pipeline %zu: %s", p, hipGetErrorString(e));
+                reset_engine(c, rc);
+                return rc;
             }
         }
+        if (oldest >= 0) {
+            rc = complete_generation(c, oldest);
+            if (rc) { reset_engine(c, rc); return rc; }
+            return 1;
+        }
+        if (++spins < 2000) std::this_thread::yield();
+        else std::this_thread::sleep_for(std::chrono::microseconds(50));
     }
-    if (!segs.empty()) {
-        OMC_TRY_DEV(transport_chunk<R>(c, segs, stats));
-        if (stats) stats->chunks++;
+}
+
+static int turn_any(omc_dev_ctx *c)
+{
+    return c->precision == OMC_PREC_FP64 ? turn<double>(c) : turn<float>(c);
+}
+
+static void stats_begin(omc_dev_ctx *c)
+{
+    memset(&c->acc, 0, sizeof c->acc);
+    for (auto &P : c->pipes) P.base = P.h->status;
+}
+
+static void stats_end(omc_dev_ctx *c, omc_dev_stats *out, int64_t histories, double t_start)
+{
+    if (!out) return;
+    omc_dev_stats S = c->acc;
+    for (auto &P : c->pipes) {
+        // the pipelines' mirrors hold the status after their last generation
+        S.photon_steps += (int64_t)(P.h->status.photon_steps - P.base.photon_steps);
+        S.electron_steps += (int64_t)(P.h->status.electron_steps - P.base.electron_steps);
+        S.photon_crossings += (int64_t)(P.h->status.photon_cross - P.base.photon_cross);
+        S.electron_crossings += (int64_t)(P.h->status.electron_cross - P.base.electron_cross);
+        S.photon_deposits += (int64_t)(P.h->status.photon_dep - P.base.photon_dep);
+        S.electron_deposits += (int64_t)(P.h->status.electron_dep - P.base.electron_dep);
+        S.killed += (int64_t)(P.h->status.killed - P.base.killed);
     }
-    if (stats) {
-        DevStatus st;
-        HIPCHK(hipMemcpy(&st, c->d_status, sizeof st, hipMemcpyDeviceToHost));
-        stats->photon_steps += (int64_t)st.photon_steps;
-        stats->electron_steps += (int64_t)st.electron_steps;
-        stats->deposits += (int64_t)st.deposits;
-        stats->killed += (int64_t)st.killed;
-        for (auto &j : jobs) stats->histories += j.nhist;
+    S.deposits = S.photon_deposits + S.electron_deposits;
+    S.histories = histories;
+    S.transport_ms = now_ms(c) - t_start;
+    *out = S;
+}
+
+// Runs the works of one synchronous call to completion (serving requests
+// already admitted keep running alongside).
+static int run_sink(omc_dev_ctx *c, RunSink &k, std::vector<Work> &works, omc_dev_stats *stats)
+{
+    k.remaining = (int)works.size();
+    for (auto &w : works) {
+        w.sink = &k;
+        w.submit_ms = now_ms(c);
+        c->waiting.push_back(&w);
     }
-    return OMC_OK;
+    stats_begin(c);
+    const double t = now_ms(c);
+    int64_t hist = 0;
+    for (auto &w : works) hist += w.nhist;
+    while (k.remaining > 0) {
+        int r = turn_any(c);
+        if (r < 0) break;                // reset_engine ended every work
+        if (r == 0 && k.remaining > 0) {
+            // admission either starts a work or fails it, so an idle engine
+            // with works left is inconsistent: stop rather than spin
+            reset_engine(c, fail(OMC_EDEVICE, "scheduler stalled with %d works left", k.remaining));
+            break;
+        }
+    }
+    stats_end(c, stats, hist, t);
+    return k.rc;
 }
 
 // ------------------------------------------------------------------ C ABI --
@@ -441,15 +909,18 @@ int omc_dev_create(omc_dev_ctx **out, const omc_dev_opts *opts)
 {
     if (!out) return fail(OMC_EARG, "null context pointer");
     *out = nullptr;
-    omc_dev_opts o = { 0, OMC_PREC_FP64, 0, 0 };
+    omc_dev_opts o = { 0, OMC_PREC_FP64, 0, 0, 0 };
     if (opts) o = *opts;
     if (o.precision != OMC_PREC_FP64 && o.precision != OMC_PREC_FP32)
         return fail(OMC_EARG, "unknown precision %d", o.precision);
+    if (o.pipelines < 0 || o.pipelines > MAX_PIPES) return fail(OMC_ERANGE, "pipelines %d (1..%d)", o.pipelines, MAX_PIPES);
+    if (o.chunk < 0) return fail(OMC_EARG, "chunk %lld", (long long)o.chunk);
     int ndev = 0;
     HIPCHK(hipGetDeviceCount(&ndev));
     if (o.device < 0 || o.device >= ndev) return fail(OMC_EDEVICE, "device %d not present (%d devices)", o.device, ndev);
     HIPCHK(hipSetDevice(o.device));
     omc_dev_ctx *c = new omc_dev_ctx();
+    c->t0 = std::chrono::steady_clock::now();
     c->device = o.device;
     c->precision = o.precision;
     hipDeviceProp_t prop;
@@ -457,22 +928,31 @@ int omc_dev_create(omc_dev_ctx **out, const omc_dev_opts *opts)
     size_t freeb = 0, totalb = 0;
     (void)hipMemGetInfo(&freeb, &totalb);
     c->budget = o.hbm_budget > 0 ? o.hbm_budget : (int64_t)(0.9 * (double)freeb);
-    size_t pb = o.precision == OMC_PREC_FP64 ? particle_bytes<double>() : particle_bytes<float>();
-    // four queues take at most a quarter of the budget, capped at 16M slots each
-    int64_t cap = c->budget / 4 / 4 / (int64_t)pb;
-    cap = std::min<int64_t>(cap, 16 << 20);
-    cap = std::max<int64_t>(cap, 1 << 16);
-    c->qcap = (uint32_t)cap;
-    c->chunk = o.chunk > 0 ? std::min<int64_t>(o.chunk, cap / 2) : cap / 4;
+    const int npipes = o.pipelines > 0 ? o.pipelines : DEFAULT_PIPES;
+    const size_t pb = o.precision == OMC_PREC_FP64 ? particle_bytes<double>() : particle_bytes<float>();
+    // live particles per pipeline, and queues four times that deep (a
+    // generation's secondaries); queues take at most half the budget
+    c->chunk = o.chunk > 0 ? o.chunk : DEFAULT_CHUNK;
+    int64_t cap = std::max<int64_t>(4 * c->chunk, 1 << 12);
+    const int64_t cap_mem = c->budget / 2 / npipes / 4 / (int64_t)pb;
+    if (cap > cap_mem) {
+        cap = cap_mem;
+        c->chunk = std::max<int64_t>(cap / 4, 1);
+    }
+    cap = std::min<int64_t>(cap, 1 << 26);
+    c->chunk = std::min<int64_t>(c->chunk, cap / 4 > 0 ? cap / 4 : 1);
     int rc = OMC_OK;
-    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) rc = fail(OMC_EDEVICE, "stream");
-    if (!rc) rc = hipEventCreate(&c->ev0) == hipSuccess && hipEventCreate(&c->ev1) == hipSuccess ? OMC_OK : fail(OMC_EDEVICE, "events");
-    if (!rc) rc = o.precision == OMC_PREC_FP64 ? alloc_queues<double>(c) : alloc_queues<float>(c);
-    if (!rc) rc = dalloc(c, (void **)&c->d_counts, 16 * sizeof(uint32_t));
-    if (!rc) rc = dalloc(c, (void **)&c->d_status, sizeof(DevStatus));
+    if (cap < 64) rc = fail(OMC_EBUDGET, "HBM budget %lld too small for the particle queues", (long long)c->budget);
+    c->qcap = (uint32_t)std::max<int64_t>(cap, 0);
+    if (!rc && hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking) != hipSuccess) rc = fail(OMC_EDEVICE, "stream");
     if (!rc) rc = dalloc(c, (void **)&c->d_jobs, MAX_JOBS * sizeof(DevJob));
-    size_t phb = o.precision == OMC_PREC_FP64 ? sizeof(DevPhantom<double>) : sizeof(DevPhantom<float>);
+    const size_t phb = o.precision == OMC_PREC_FP64 ? sizeof(DevPhantom<double>) : sizeof(DevPhantom<float>);
     if (!rc) rc = dalloc(c, &c->d_phantoms, MAX_PHANTOMS * phb);
+    c->pipes.resize(rc ? 0 : npipes);
+    for (auto &P : c->pipes) {
+        if (rc) break;
+        rc = o.precision == OMC_PREC_FP64 ? alloc_pipe<double>(c, P) : alloc_pipe<float>(c, P);
+    }
     if (rc) { omc_dev_destroy(c); return rc; }
     *out = c;
     return OMC_OK;
@@ -482,18 +962,19 @@ int omc_dev_destroy(omc_dev_ctx *c)
 {
     if (!c) return OMC_OK;
     (void)hipSetDevice(c->device);
-    if (c->stream) (void)hipStreamSynchronize(c->stream);
-    for (auto &qq : c->q) for (auto &p : qq) if (p) { (void)hipFree(p); p = nullptr; }
+    for (auto &P : c->pipes) free_pipe(P);
+    if (c->aux) (void)hipStreamSynchronize(c->aux);
+    for (int j = 0; j < c->jobs_hi; ++j) if (c->jobs[j].used) release_job(c, j);
+    for (auto &kv : c->pool) (void)hipFree(kv.second);
     void *ptrs[] = { c->small, c->ph_mfp, c->ph_br, c->ph_pmax, c->el, c->csda, c->spin, c->ms,
-                     c->d_counts, c->d_status, c->d_jobs, c->d_phantoms };
+                     c->d_jobs, c->d_phantoms, c->fin_sums };
     for (void *p : ptrs) if (p) (void)hipFree(p);
     for (auto &P : c->ph) {
-        DevBuf *bufs[] = { &P.xb, &P.yb, &P.zb, &P.med, &P.rhof };
+        DevBuf *bufs[] = { &P.xb, &P.yb, &P.zb, &P.med, &P.rhof, &P.rho };
         for (auto *b : bufs) if (b->p) (void)hipFree(b->p);
     }
-    if (c->ev0) (void)hipEventDestroy(c->ev0);
-    if (c->ev1) (void)hipEventDestroy(c->ev1);
-    if (c->stream) (void)hipStreamDestroy(c->stream);
+    for (auto &kv : c->tickets) delete kv.second;
+    if (c->aux) (void)hipStreamDestroy(c->aux);
     delete c;
     return OMC_OK;
 }
@@ -504,6 +985,7 @@ int omc_dev_upload_media(omc_dev_ctx *c, const omc_tables *t)
 {
     if (!c || !t) return fail(OMC_EARG, "null argument");
     if (t->nmed < 1 || t->nmed > OMC_MXMED) return fail(OMC_ERANGE, "nmed %d", t->nmed);
+    if (any_running(c) || !c->waiting.empty()) return fail(OMC_EARG, "media cannot change while work is queued");
     HIPCHK(hipSetDevice(c->device));
     return c->precision == OMC_PREC_FP64 ? upload_tables<double>(c, t) : upload_tables<float>(c, t);
 }
@@ -518,24 +1000,87 @@ int omc_dev_upload_phantom(omc_dev_ctx *c, const omc_phantom *ph, const omc_regi
 int omc_dev_release_phantom(omc_dev_ctx *c, int id)
 {
     if (!c || id < 0 || id >= MAX_PHANTOMS || !c->ph[id].used) return fail(OMC_EARG, "bad phantom id %d", id);
+    for (int j = 0; j < c->jobs_hi; ++j)
+        if (c->jobs[j].used && c->jobs[j].w->phantom == id) return fail(OMC_EARG, "phantom %d is in use", id);
+    for (auto *w : c->waiting)
+        if (w->phantom == id) return fail(OMC_EARG, "phantom %d is in use", id);
     PhantomSlot &P = c->ph[id];
-    DevBuf *bufs[] = { &P.xb, &P.yb, &P.zb, &P.med, &P.rhof };
+    DevBuf *bufs[] = { &P.xb, &P.yb, &P.zb, &P.med, &P.rhof, &P.rho };
     for (auto *b : bufs) { dfree(c, b->p, b->bytes); b->p = nullptr; b->bytes = 0; }
     P.used = false;
+    return OMC_OK;
+}
+
+static int check_phantom(omc_dev_ctx *c, int id)
+{
+    if (!c->media) return fail(OMC_EARG, "media tables not uploaded");
+    if (id < 0 || id >= MAX_PHANTOMS || !c->ph[id].used) return fail(OMC_EARG, "bad phantom id %d", id);
     return OMC_OK;
 }
 
 static int check_run(omc_dev_ctx *c, const omc_dev_run_params *rp)
 {
     if (!c || !rp) return fail(OMC_EARG, "null argument");
-    if (!c->media) return fail(OMC_EARG, "media tables not uploaded");
-    if (rp->phantom_id < 0 || rp->phantom_id >= MAX_PHANTOMS || !c->ph[rp->phantom_id].used)
-        return fail(OMC_EARG, "bad phantom id %d", rp->phantom_id);
+    OMC_TRY_DEV(check_phantom(c, rp->phantom_id));
     if (rp->nsplit != 1)
         return fail(OMC_ENOTSUP, "photon splitting (nsplit=%d) is not supported by the device path", rp->nsplit);
-    if (rp->nbatch < 1 || rp->nperbatch < 1) return fail(OMC_EARG, "nbatch and nperbatch must be >= 1");
-    if ((int64_t)rp->nbatch * rp->nperbatch > 0xffffffffLL)
-        return fail(OMC_ERANGE, "more than 2^32 histories in one run");
+    if (rp->nbatch < 1) return fail(OMC_EARG, "nbatch must be >= 1");
+    return OMC_OK;
+}
+
+static void source_fields(DevJob &j, const omc_spectrum &spec, int charge)
+{
+    j.charge = charge;
+    j.spectrum = spec.spectrum;
+    j.energy = spec.energy;
+    j.deltak = spec.deltak;
+}
+
+static void dosxyz_work(Work &w, const omc_source_dosxyz *src)
+{
+    memset(&w.tmpl, 0, sizeof w.tmpl);
+    w.tmpl.kind = 0;
+    source_fields(w.tmpl, src->spec, src->charge);
+    w.tmpl.ssd = src->ssd; w.tmpl.xinl = src->xinl; w.tmpl.yinl = src->yinl;
+    w.tmpl.xsize = src->xsize; w.tmpl.ysize = src->ysize;
+    w.tmpl.ixinl = src->ixinl; w.tmpl.iyinl = src->iyinl;
+    w.cdf.assign(src->spec.cdfinv1, src->spec.cdfinv1 + OMC_INVDIM);
+    w.cdf.insert(w.cdf.end(), src->spec.cdfinv2, src->spec.cdfinv2 + OMC_INVDIM);
+}
+
+static int beamlet_work(Work &w, const omc_source_beamlets *src, int b)
+{
+    if (b < 0 || b >= src->nbeamlets) return fail(OMC_ERANGE, "beamlet %d of %d", b, src->nbeamlets);
+    const int ib = src->ibeam[b];
+    if (ib < 0 || ib >= src->nbeams) return fail(OMC_ERANGE, "beamlet %d: beam %d of %d", b, ib, src->nbeams);
+    memset(&w.tmpl, 0, sizeof w.tmpl);
+    w.tmpl.kind = 1;
+    source_fields(w.tmpl, src->spec, src->charge);
+    for (int d = 0; d < 3; ++d) {
+        w.tmpl.src[d] = src->src[3 * ib + d];
+        w.tmpl.corner[d] = src->corner[3 * b + d];
+        w.tmpl.side1[d] = src->side1[3 * b + d];
+        w.tmpl.side2[d] = src->side2[3 * b + d];
+    }
+    w.cdf.assign(src->spec.cdfinv1, src->spec.cdfinv1 + OMC_INVDIM);
+    w.cdf.insert(w.cdf.end(), src->spec.cdfinv2, src->spec.cdfinv2 + OMC_INVDIM);
+    return OMC_OK;
+}
+
+static void stream_fields(Work &w, const uint32_t seed[2], int64_t hist0)
+{
+    w.tmpl.hist0 = (uint64_t)hist0;
+    w.tmpl.s0 = seed[0];
+    w.tmpl.s1 = seed[1];
+}
+
+static int set_histories(Work &w, int64_t nperbatch, int nbatch)
+{
+    if (nperbatch < 1) nperbatch = 1;
+    if (nperbatch * (int64_t)nbatch > 0xffffffffLL) return fail(OMC_ERANGE, "more than 2^32 histories in one job");
+    w.nperbatch = nperbatch;
+    w.nbatch = nbatch;
+    w.nhist = nperbatch * nbatch;
     return OMC_OK;
 }
 
@@ -545,59 +1090,169 @@ int omc_dev_run_dosxyz(omc_dev_ctx *c, const omc_source_dosxyz *src, const omc_d
     OMC_TRY_DEV(check_run(c, rp));
     if (!src || !sum || !sum2) return fail(OMC_EARG, "null argument");
     if (src->charge < -1 || src->charge > 1) return fail(OMC_EARG, "charge %d", src->charge);
+    if (rp->nperbatch < 1) return fail(OMC_EARG, "nperbatch must be >= 1");
     HIPCHK(hipSetDevice(c->device));
-    const PhantomSlot &P = c->ph[rp->phantom_id];
-    const int64_t nreg = P.nreg;
-    const int64_t nhist = rp->nperbatch * (int64_t)rp->nbatch;
-    double *tally = nullptr, *cdf = nullptr, *led = nullptr, *fs = nullptr;
-    size_t tb = sizeof(double) * (size_t)nreg * rp->nbatch;
-    size_t lb = rp->ledger ? sizeof(double) * 3 * (size_t)nhist : 0;
-    int rc = dalloc(c, (void **)&tally, tb);
-    if (!rc) rc = dalloc(c, (void **)&cdf, 2 * OMC_INVDIM * sizeof(double));
-    if (!rc && lb) rc = dalloc(c, (void **)&led, lb);
-    if (!rc) rc = dalloc(c, (void **)&fs, 2 * sizeof(double) * (size_t)nreg);
-    if (!rc) {
-        launch_zero_f64(tally, (int64_t)nreg * rp->nbatch, c->stream);
-        if (led) launch_zero_f64(led, 3 * nhist, c->stream);
-        std::vector<double> cd(2 * OMC_INVDIM);
-        memcpy(cd.data(), src->spec.cdfinv1, OMC_INVDIM * sizeof(double));
-        memcpy(cd.data() + OMC_INVDIM, src->spec.cdfinv2, OMC_INVDIM * sizeof(double));
-        if (hipMemcpy(cdf, cd.data(), cd.size() * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
-            rc = fail(OMC_EDEVICE, "copy spectrum");
+    std::vector<Work> works(1);
+    Work &w = works[0];
+    dosxyz_work(w, src);
+    stream_fields(w, rp->seed, rp->hist0);
+    OMC_TRY_DEV(set_histories(w, rp->nperbatch, rp->nbatch));
+    w.phantom = rp->phantom_id;
+    w.nreg = c->ph[rp->phantom_id].nreg;
+    w.ledger = rp->ledger != nullptr;
+    RunSink k;
+    k.sum = sum;
+    k.sum2 = sum2;
+    k.ledger = rp->ledger;
+    return run_sink(c, k, works, rp->stats);
+}
+
+static int beamlet_works(omc_dev_ctx *c, const omc_source_beamlets *src, const int64_t *nhist,
+                         const omc_dev_run_params *rp, std::vector<Work> &works)
+{
+    OMC_TRY_DEV(check_run(c, rp));
+    if (!src || !nhist) return fail(OMC_EARG, "null argument");
+    if (src->nbeamlets < 1 || !src->ibeam || !src->src || !src->corner || !src->side1 || !src->side2)
+        return fail(OMC_EARG, "empty beamlet source");
+    if (src->charge < -1 || src->charge > 1) return fail(OMC_EARG, "charge %d", src->charge);
+    if (rp->ledger) return fail(OMC_EARG, "the per-history ledger is only available for dosxyz runs");
+    works.resize(src->nbeamlets);
+    int64_t h0 = rp->hist0;
+    for (int b = 0; b < src->nbeamlets; ++b) {
+        Work &w = works[b];
+        if (nhist[b] < 1) return fail(OMC_EARG, "beamlet %d: %lld histories", b, (long long)nhist[b]);
+        OMC_TRY_DEV(beamlet_work(w, src, b));
+        OMC_TRY_DEV(set_histories(w, nhist[b] / rp->nbatch, rp->nbatch));
+        stream_fields(w, rp->seed, h0);
+        h0 += w.nhist;
+        w.index = b;
+        w.phantom = rp->phantom_id;
+        w.nreg = c->ph[rp->phantom_id].nreg;
     }
-    if (!rc) {
-        RunJob j;
-        memset(&j, 0, sizeof j);
-        j.j.phantom = rp->phantom_id;
-        j.j.kind = 0;
-        fill_source_job(j.j, src->spec, src->charge, cdf);
-        j.j.ssd = src->ssd; j.j.xinl = src->xinl; j.j.yinl = src->yinl;
-        j.j.xsize = src->xsize; j.j.ysize = src->ysize;
-        j.j.ixinl = src->ixinl; j.j.iyinl = src->iyinl;
-        j.j.hist0 = (uint64_t)rp->hist0;
-        j.j.nperbatch = rp->nperbatch;
-        j.j.s0 = rp->seed[0]; j.j.s1 = rp->seed[1];
-        j.j.tally = tally;
-        j.j.ledger = led;
-        j.nhist = nhist;
-        std::vector<RunJob> jobs(1, j);
-        rc = c->precision == OMC_PREC_FP64 ? run_jobs<double>(c, jobs, rp->stats) : run_jobs<float>(c, jobs, rp->stats);
+    return OMC_OK;
+}
+
+int omc_dev_run_beamlets(omc_dev_ctx *c, const omc_source_beamlets *src, const int64_t *nhist,
+                         const omc_dev_run_params *rp, double *sum, double *sum2)
+{
+    if (!sum || !sum2) return fail(OMC_EARG, "null argument");
+    std::vector<Work> works;
+    OMC_TRY_DEV(beamlet_works(c, src, nhist, rp, works));
+    HIPCHK(hipSetDevice(c->device));
+    RunSink k;
+    k.beamlets = true;
+    k.sum = sum;
+    k.sum2 = sum2;
+    return run_sink(c, k, works, rp->stats);
+}
+
+int omc_dev_beamlets_sparse(omc_dev_ctx *c, const omc_source_beamlets *src, const int64_t *nhist,
+                            const omc_dev_run_params *rp, double rel_thr, int64_t *jcs, int64_t *irs,
+                            double *vals, int64_t cap, int64_t *nnz)
+{
+    if (!c || !jcs || !nnz || (cap > 0 && (!irs || !vals))) return fail(OMC_EARG, "null argument");
+    if (src) {
+        std::vector<Work> works;
+        OMC_TRY_DEV(beamlet_works(c, src, nhist, rp, works));
+        HIPCHK(hipSetDevice(c->device));
+        RunSink k;
+        k.beamlets = true;
+        k.sparse = true;
+        k.rel_thr = rel_thr;
+        k.rows.resize(works.size());
+        k.vals.resize(works.size());
+        c->kept_rows.clear();
+        c->kept_vals.clear();
+        OMC_TRY_DEV(run_sink(c, k, works, rp->stats));
+        c->kept_rows.swap(k.rows);
+        c->kept_vals.swap(k.vals);
+    } else if (c->kept_rows.empty()) {
+        return fail(OMC_EARG, "no kept sparse matrix to copy");
     }
-    if (!rc) {
-        launch_fold(tally, rp->nbatch, nreg, fs, fs + nreg, c->stream);
-        std::vector<double> h(2 * (size_t)nreg);
-        if (hipMemcpyAsync(h.data(), fs, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream) != hipSuccess ||
-            hipStreamSynchronize(c->stream) != hipSuccess)
-            rc = fail(OMC_EDEVICE, "copy tallies");
-        for (int64_t i = 0; !rc && i < nreg; ++i) { sum[i] += h[i]; sum2[i] += h[nreg + i]; }
-        if (!rc && led && hipMemcpy(rp->ledger, led, lb, hipMemcpyDeviceToHost) != hipSuccess)
-            rc = fail(OMC_EDEVICE, "copy ledger");
+    int64_t total = 0;
+    for (auto &r : c->kept_rows) total += (int64_t)r.size();
+    *nnz = total;
+    if (total > cap) return fail(OMC_ERANGE, "sparse Dij has %lld entries, capacity %lld", (long long)total, (long long)cap);
+    int64_t at = 0;
+    jcs[0] = 0;
+    for (size_t b = 0; b < c->kept_rows.size(); ++b) {
+        std::copy(c->kept_rows[b].begin(), c->kept_rows[b].end(), irs + at);
+        std::copy(c->kept_vals[b].begin(), c->kept_vals[b].end(), vals + at);
+        at += (int64_t)c->kept_rows[b].size();
+        jcs[b + 1] = at;
     }
-    dfree(c, tally, tb);
-    dfree(c, cdf, 2 * OMC_INVDIM * sizeof(double));
-    dfree(c, led, lb);
-    dfree(c, fs, 2 * sizeof(double) * (size_t)nreg);
-    return rc;
+    c->kept_rows.clear();
+    c->kept_vals.clear();
+    return OMC_OK;
+}
+
+int omc_dev_submit(omc_dev_ctx *c, const omc_dev_request *req)
+{
+    if (!c || !req) return fail(OMC_EARG, "null argument");
+    OMC_TRY_DEV(check_phantom(c, req->phantom_id));
+    if (req->nhist < 1 || req->nbatch < 1) return fail(OMC_EARG, "nhist and nbatch must be >= 1");
+    Work *w = new Work();
+    int rc = OMC_OK;
+    if (req->kind == 0) {
+        if (!req->src_dosxyz) rc = fail(OMC_EARG, "dosxyz request without a source");
+        else if (req->src_dosxyz->charge < -1 || req->src_dosxyz->charge > 1) rc = fail(OMC_EARG, "charge");
+        else dosxyz_work(*w, req->src_dosxyz);
+    } else if (req->kind == 1) {
+        if (!req->src_beamlets) rc = fail(OMC_EARG, "beamlet request without a source");
+        else rc = beamlet_work(*w, req->src_beamlets, req->beamlet);
+    } else {
+        rc = fail(OMC_EARG, "request kind %d", req->kind);
+    }
+    if (!rc) rc = set_histories(*w, req->nhist / req->nbatch, req->nbatch);
+    if (rc) { delete w; return rc; }
+    stream_fields(*w, req->seed, 0);
+    w->phantom = req->phantom_id;
+    w->nreg = c->ph[req->phantom_id].nreg;
+    w->ticket = c->next_ticket++;
+    w->submit_ms = now_ms(c);
+    c->tickets[w->ticket] = w;
+    c->waiting.push_back(w);
+    c->unfinished++;
+    return w->ticket;
+}
+
+int omc_dev_step(omc_dev_ctx *c)
+{
+    if (!c) return fail(OMC_EARG, "null argument");
+    HIPCHK(hipSetDevice(c->device));
+    int r = turn_any(c);
+    if (r < 0) return r;
+    if (r == 0 && c->unfinished > 0) {
+        // requests can only wait while something runs (see admit)
+        int rc = fail(OMC_EDEVICE, "scheduler stalled with %d requests left", c->unfinished);
+        reset_engine(c, rc);
+        return rc;
+    }
+    return c->unfinished;
+}
+
+int omc_dev_poll(omc_dev_ctx *c, int ticket, omc_dev_result *res, double *sum, double *sum2)
+{
+    if (!c || !res) return fail(OMC_EARG, "null argument");
+    auto it = c->tickets.find(ticket);
+    if (it == c->tickets.end()) return fail(OMC_EARG, "unknown ticket %d", ticket);
+    Work *w = it->second;
+    memset(res, 0, sizeof *res);
+    res->ticket = ticket;
+    res->done = w->state == W_DONE || w->state == W_FAILED;
+    res->status = w->status;
+    res->submit_ms = w->submit_ms;
+    res->admit_ms = w->admit_ms;
+    res->finish_ms = w->finish_ms;
+    res->nhist = w->nhist;
+    if (!res->done) return OMC_OK;
+    if (w->state == W_DONE) {
+        if (sum) std::copy(w->sum.begin(), w->sum.end(), sum);
+        if (sum2) std::copy(w->sum2.begin(), w->sum2.end(), sum2);
+    }
+    c->tickets.erase(it);
+    delete w;
+    return OMC_OK;
 }
 
 }  // extern "C"

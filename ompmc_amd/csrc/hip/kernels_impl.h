@@ -9,17 +9,26 @@ namespace omc {
 
 // initHistory for a dosxyz field (omc_dosxyz.c:964-1068) or a matRad beamlet
 // (omc_matrad.c:1084-1254): one thread per history, primary pushed to the
-// photon or charged-particle queue.
+// photon or charged-particle queue.  One launch serves `nseg` segments of
+// consecutive histories, possibly of different jobs; segs[].off is the
+// exclusive prefix of the segment sizes.
 template <typename R>
-__global__ void __launch_bounds__(256) k_source(const DevJob *jobs, const DevPhantom<R> *phs, int job,
-                                                uint64_t first, uint32_t n, Queue<R> qp, Queue<R> qe,
-                                                DevStatus *st)
+__global__ void __launch_bounds__(256) k_source(const DevJob *jobs, const DevPhantom<R> *phs,
+                                                const SourceSeg *segs, int nseg, uint32_t ntot,
+                                                Queue<R> qp, Queue<R> qe, DevStatus *st)
 {
     uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
+    if (t >= ntot) return;
+    int lo = 0, hi = nseg - 1;             // last segment with off <= t
+    while (lo < hi) {
+        int mid = (lo + hi + 1) >> 1;
+        if (segs[mid].off <= t) lo = mid; else hi = mid - 1;
+    }
+    const SourceSeg sg = segs[lo];
+    const int job = (int)sg.job;
     const DevJob &J = jobs[job];
     const DevPhantom<R> &g = phs[J.phantom];
-    const uint64_t h = first + t;
+    const uint64_t h = sg.first + (t - sg.off);
     Particle<R> p;
     p.job = (uint32_t)job;
     p.hist = (uint32_t)h;
@@ -111,11 +120,12 @@ __global__ void __launch_bounds__(256) k_source(const DevJob *jobs, const DevPha
 }
 
 template <typename R>
-void Launch<R>::source(const DevJob *jobs, const DevPhantom<R> *ph, int job, uint64_t first, uint32_t n,
-                       const Queue<R> &qp, const Queue<R> &qe, DevStatus *st, hipStream_t s)
+void Launch<R>::source(const DevJob *jobs, const DevPhantom<R> *ph, const SourceSeg *segs, int nseg,
+                       uint32_t ntot, const Queue<R> &qp, const Queue<R> &qe, DevStatus *st, hipStream_t s)
 {
-    if (!n) return;
-    hipLaunchKernelGGL(k_source<R>, dim3((n + 255) / 256), dim3(256), 0, s, jobs, ph, job, first, n, qp, qe, st);
+    if (!ntot || nseg < 1) return;
+    hipLaunchKernelGGL(k_source<R>, dim3((ntot + 255) / 256), dim3(256), 0, s, jobs, ph, segs, nseg, ntot, qp, qe,
+                       st);
 }
 
 template <typename R>

@@ -26,8 +26,8 @@ template <typename R> struct KArgs {
     const DevPhantom<R> *phantoms;
     const DevJob *jobs;
     Queue<R> qin;            // particles consumed by this launch
-    uint32_t in_begin, in_end;
-    uint32_t *fetch;         // fetch counter (starts at in_begin)
+    const uint32_t *in_count;    // their number, final before the launch starts
+    uint32_t *fetch;         // fetch counter (zero at launch)
     Queue<R> qp;             // photon output queue
     Queue<R> qe;             // charged-particle output queue
     DevStatus *status;
@@ -103,7 +103,7 @@ template <typename R> struct PhotonLane {
 // sets in motion are pushed to A.qe.
 template <typename R>
 __device__ bool photon_iteration(const KArgs<R> &A, Particle<R> &p, PhotonLane<R> &s, LaneGeo<R> &G,
-                                 Scorer<R> &sc, uint32_t &nsteps)
+                                 Scorer<R> &sc, uint32_t &nsteps, uint32_t &ncross)
 {
     const DevTables<R> &T = A.T;
     if (s.phase == 0) {
@@ -154,6 +154,7 @@ __device__ bool photon_iteration(const KArgs<R> &A, Particle<R> &p, PhotonLane<R
     p.z += ustep * p.w;
     if (s.imed != -1) s.dpmfp = m_fmax((R)0.0, s.dpmfp - ustep / s.gmfp);
     if (irnew != irl) {
+        ncross++;
         p.ir = irnew;
         s.imed = G.medium(irnew);
     }
@@ -217,9 +218,68 @@ __device__ __forceinline__ void electron_cutoff(const KArgs<R> &A, Particle<R> &
 // One ustep iteration of electron() (ompmc.c:4765-5341) plus, where the
 // reference reaches them, the tstep-loop head (4694-4763), the fictitious
 // rejection (5347-5372) and the discrete interaction (5375-5429).
+// End of the ustep loop (ompmc.c:5347-5433): fictitious-cross-section
+// rejection, then the discrete interaction.
+template <typename R>
+__device__ __forceinline__ int electron_interaction(const KArgs<R> &A, Particle<R> &p, ElectronLane<R> &s)
+{
+    const DevTables<R> &T = A.T;
+    const int iq = p.iq;
+    const int qel = (1 + iq) / 2;
+    // ---- end of the ustep loop: fictitious-cross-section rejection ----
+    {
+        const ElecBin<R> &eb = T.el[(s.imed * 2 + qel) * MXEKE + s.lelke];
+        const ElecCsda &cs = T.csda[(s.imed * 2 + qel) * MXEKE + s.lelke];
+        R sigf = s.elke * eb.sig1 + eb.sig0;
+        R dedx0 = s.elke * (R)cs.dedx1 + (R)cs.dedx0;
+        sigf /= dedx0;
+        R sigratio = sigf / s.sig0;
+        R rfict = urand<R>(p.rng);
+        if (rfict >= sigratio) { s.phase = 1; return E_ALIVE; }
+    }
+
+    // ---- discrete interaction ----
+    const int m = s.imed;
+    const ElecBin<R> &eb = T.el[(m * 2 + qel) * MXEKE + s.lelke];
+    Particle<R> b;
+    s.phase = 0;
+    if (iq < 0) {
+        R ebr1 = s.elke * eb.br11 + eb.br10;
+        R rnno = urand<R>(p.rng);
+        if (rnno <= ebr1) {
+            brems<R>(T, p, b, m);
+            push<R>(A.qp, b, A.status);
+        } else if (p.e <= (R)T.small->med[m].thmoll) {
+            if (ebr1 <= (R)0) return E_ALIVE;   // neither process possible: re-enter
+            brems<R>(T, p, b, m);
+            push<R>(A.qp, b, A.status);
+        } else {
+            if (moller<R>(T, p, b, m)) push<R>(A.qe, b, A.status);
+        }
+        return E_ALIVE;
+    }
+    R pbr1 = s.elke * eb.br11 + eb.br10;
+    R rnno = urand<R>(p.rng);
+    if (rnno < pbr1) {
+        brems<R>(T, p, b, m);
+        push<R>(A.qp, b, A.status);
+        return E_ALIVE;
+    }
+    R pbr2 = s.elke * eb.br21 + eb.br20;
+    if (rnno < pbr2) {
+        bhabha<R>(T, p, b, m);
+        push<R>(A.qe, b, A.status);
+        return E_ALIVE;
+    }
+    annih<R>(p, b);
+    push<R>(A.qp, p, A.status);
+    push<R>(A.qp, b, A.status);
+    return E_DONE;
+}
+
 template <typename R>
 __device__ int electron_iteration(const KArgs<R> &A, Particle<R> &p, ElectronLane<R> &s, LaneGeo<R> &G,
-                                  Scorer<R> &sc, uint32_t &nsteps)
+                                  Scorer<R> &sc, uint32_t &nsteps, uint32_t &ncross)
 {
     const DevTables<R> &T = A.T;
     const R rm = (R)RM;
@@ -394,13 +454,21 @@ __device__ int electron_iteration(const KArgs<R> &A, Particle<R> &p, ElectronLan
             p.z += p.w * vstep;
         }
         if (irnew != irold) {
+            ncross++;
             p.ir = irnew;
             irl = irnew;
             s.imed = G.medium(irl);
         }
         if (eie <= (R)G.g->ecut[s.imed + 1]) { electron_cutoff<R>(A, p, G, sc, eie); return E_DONE; }
-        s.phase = 1;                         // break -> continue tstep loop
-        return E_ALIVE;
+        // leave the ustep loop; the tstep loop restarts unless a step through
+        // vacuum brought the particle back into the medium it last lost energy
+        // in, where the reference goes on to the fictitious-sigma test
+        // (ompmc.c:5347; medold is only updated by steps in matter)
+        if (s.imed != s.medold || ustep == (R)0.0 || s.imed == -1) {
+            s.phase = 1;
+            return E_ALIVE;
+        }
+        return electron_interaction<R>(A, p, s);
     }
     R vstep = ustep;
     const R rhof = s.rhof;
@@ -457,6 +525,7 @@ __device__ int electron_iteration(const KArgs<R> &A, Particle<R> &p, ElectronLan
         s.lelke = pwlf_idx(s.elke, M->eke1, M->eke0) - 1;
     }
     if (irnew != irold) {
+        ncross++;
         p.ir = irnew;
         irl = irnew;
         s.imed = G.medium(irl);
@@ -468,55 +537,7 @@ __device__ int electron_iteration(const KArgs<R> &A, Particle<R> &p, ElectronLan
     if (total_tstep < 1.0E-9) s.demfp = (R)0.0;
     if (s.demfp >= (R)EPSEMFP) return E_ALIVE;      // next ustep iteration
 
-    // ---- end of the ustep loop: fictitious-cross-section rejection ----
-    {
-        const ElecBin<R> &eb = T.el[(s.imed * 2 + qel) * MXEKE + s.lelke];
-        const ElecCsda &cs = T.csda[(s.imed * 2 + qel) * MXEKE + s.lelke];
-        R sigf = s.elke * eb.sig1 + eb.sig0;
-        R dedx0 = s.elke * (R)cs.dedx1 + (R)cs.dedx0;
-        sigf /= dedx0;
-        R sigratio = sigf / s.sig0;
-        R rfict = urand<R>(p.rng);
-        if (rfict >= sigratio) { s.phase = 1; return E_ALIVE; }
-    }
-
-    // ---- discrete interaction ----
-    const int m = s.imed;
-    const ElecBin<R> &eb = T.el[(m * 2 + qel) * MXEKE + s.lelke];
-    Particle<R> b;
-    s.phase = 0;
-    if (iq < 0) {
-        R ebr1 = s.elke * eb.br11 + eb.br10;
-        R rnno = urand<R>(p.rng);
-        if (rnno <= ebr1) {
-            brems<R>(T, p, b, m);
-            push<R>(A.qp, b, A.status);
-        } else if (p.e <= (R)T.small->med[m].thmoll) {
-            if (ebr1 <= (R)0) return E_ALIVE;   // neither process possible: re-enter
-            brems<R>(T, p, b, m);
-            push<R>(A.qp, b, A.status);
-        } else {
-            if (moller<R>(T, p, b, m)) push<R>(A.qe, b, A.status);
-        }
-        return E_ALIVE;
-    }
-    R pbr1 = s.elke * eb.br11 + eb.br10;
-    R rnno = urand<R>(p.rng);
-    if (rnno < pbr1) {
-        brems<R>(T, p, b, m);
-        push<R>(A.qp, b, A.status);
-        return E_ALIVE;
-    }
-    R pbr2 = s.elke * eb.br21 + eb.br20;
-    if (rnno < pbr2) {
-        bhabha<R>(T, p, b, m);
-        push<R>(A.qe, b, A.status);
-        return E_ALIVE;
-    }
-    annih<R>(p, b);
-    push<R>(A.qp, p, A.status);
-    push<R>(A.qp, b, A.status);
-    return E_DONE;
+    return electron_interaction<R>(A, p, s);
 }
 
 // ============================================================ kernels =====
@@ -528,11 +549,12 @@ __global__ void __launch_bounds__(256) k_photon(KArgs<R> A)
     LaneGeo<R> G;
     Scorer<R> sc;
     bool alive = false, exhausted = false;
-    uint32_t nsteps = 0, iters = 0;
+    uint32_t nsteps = 0, ncross = 0, iters = 0;
+    const uint32_t in_end = min(*A.in_count, A.qin.cap);
     for (;;) {
         if (!alive && !exhausted) {
             uint32_t i = wave_reserve(A.fetch, 1);
-            if (i < A.in_end) {
+            if (i < in_end) {
                 q_load(A.qin, i, p);
                 G.bind(A, p);
                 s.phase = 0;
@@ -543,7 +565,7 @@ __global__ void __launch_bounds__(256) k_photon(KArgs<R> A)
         }
         if (__ballot(alive) == 0) break;
         if (alive) {
-            alive = photon_iteration<R>(A, p, s, G, sc, nsteps);
+            alive = photon_iteration<R>(A, p, s, G, sc, nsteps, ncross);
             if (alive && ++iters > MAX_ITERS) {         // watchdog: no lane runs forever
                 atomicAdd(&A.status->killed, 1u);
                 alive = false;
@@ -553,7 +575,8 @@ __global__ void __launch_bounds__(256) k_photon(KArgs<R> A)
     }
     sc.flush(G);
     atomicAdd(&A.status->photon_steps, (unsigned long long)nsteps);
-    atomicAdd(&A.status->deposits, (unsigned long long)sc.ndep);
+    atomicAdd(&A.status->photon_cross, (unsigned long long)ncross);
+    atomicAdd(&A.status->photon_dep, (unsigned long long)sc.ndep);
 }
 
 template <typename R>
@@ -564,11 +587,12 @@ __global__ void __launch_bounds__(256) k_electron(KArgs<R> A)
     LaneGeo<R> G;
     Scorer<R> sc;
     bool alive = false, exhausted = false;
-    uint32_t nsteps = 0, iters = 0;
+    uint32_t nsteps = 0, ncross = 0, iters = 0;
+    const uint32_t in_end = min(*A.in_count, A.qin.cap);
     for (;;) {
         if (!alive && !exhausted) {
             uint32_t i = wave_reserve(A.fetch, 1);
-            if (i < A.in_end) {
+            if (i < in_end) {
                 q_load(A.qin, i, p);
                 G.bind(A, p);
                 s.phase = 0;
@@ -579,7 +603,7 @@ __global__ void __launch_bounds__(256) k_electron(KArgs<R> A)
         }
         if (__ballot(alive) == 0) break;
         if (alive) {
-            int r = electron_iteration<R>(A, p, s, G, sc, nsteps);
+            int r = electron_iteration<R>(A, p, s, G, sc, nsteps, ncross);
             if (r != E_DONE && ++iters > MAX_ITERS) {   // watchdog: no lane runs forever
                 atomicAdd(&A.status->killed, 1u);
                 r = E_DONE;
@@ -593,7 +617,8 @@ __global__ void __launch_bounds__(256) k_electron(KArgs<R> A)
     }
     sc.flush(G);
     atomicAdd(&A.status->electron_steps, (unsigned long long)nsteps);
-    atomicAdd(&A.status->deposits, (unsigned long long)sc.ndep);
+    atomicAdd(&A.status->electron_cross, (unsigned long long)ncross);
+    atomicAdd(&A.status->electron_dep, (unsigned long long)sc.ndep);
 }
 
 }  // namespace omc

@@ -14,16 +14,19 @@ _dp = C.POINTER(C.c_double)
 
 
 class DevOpts(C.Structure):
-    _fields_ = [("device", C.c_int), ("precision", C.c_int), ("hbm_budget", C.c_int64), ("chunk", C.c_int64)]
+    _fields_ = [("device", C.c_int), ("precision", C.c_int), ("hbm_budget", C.c_int64), ("chunk", C.c_int64),
+                ("pipelines", C.c_int)]
 
 
 class DevStats(C.Structure):
     _fields_ = [("histories", C.c_int64), ("photon_steps", C.c_int64), ("electron_steps", C.c_int64),
                 ("deposits", C.c_int64), ("photons_pushed", C.c_int64), ("electrons_pushed", C.c_int64),
-                ("generations", C.c_int64), ("chunks", C.c_int64), ("transport_ms", C.c_double),
+                ("generations", C.c_int64), ("injections", C.c_int64), ("transport_ms", C.c_double),
                 ("photon_kernel_ms", C.c_double), ("electron_kernel_ms", C.c_double),
                 ("source_kernel_ms", C.c_double), ("photon_launches", C.c_int64),
-                ("electron_launches", C.c_int64), ("killed", C.c_int64)]
+                ("electron_launches", C.c_int64), ("killed", C.c_int64), ("photon_crossings", C.c_int64),
+                ("electron_crossings", C.c_int64), ("photon_deposits", C.c_int64),
+                ("electron_deposits", C.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -42,8 +45,8 @@ class Request(C.Structure):
 
 
 class Result(C.Structure):
-    _fields_ = [("ticket", C.c_int), ("done", C.c_int), ("submit_ms", C.c_double), ("admit_ms", C.c_double),
-                ("finish_ms", C.c_double), ("nhist", C.c_int64)]
+    _fields_ = [("ticket", C.c_int), ("done", C.c_int), ("status", C.c_int), ("submit_ms", C.c_double),
+                ("admit_ms", C.c_double), ("finish_ms", C.c_double), ("nhist", C.c_int64)]
 
 
 _lib = None
@@ -65,6 +68,15 @@ def lib():
                                              C.POINTER(C.c_int)]
         L.omc_dev_release_phantom.argtypes = [C.c_void_p, C.c_int]
         L.omc_dev_run_dosxyz.argtypes = [C.c_void_p, C.POINTER(N.SourceDosxyz), C.POINTER(RunParams), _dp, _dp]
+        _i64p = C.POINTER(C.c_int64)
+        L.omc_dev_run_beamlets.argtypes = [C.c_void_p, C.POINTER(N.SourceBeamlets), _i64p, C.POINTER(RunParams),
+                                           _dp, _dp]
+        L.omc_dev_beamlets_sparse.argtypes = [C.c_void_p, C.POINTER(N.SourceBeamlets), _i64p,
+                                              C.POINTER(RunParams), C.c_double, _i64p, _i64p, _dp, C.c_int64,
+                                              _i64p]
+        L.omc_dev_submit.argtypes = [C.c_void_p, C.POINTER(Request)]
+        L.omc_dev_step.argtypes = [C.c_void_p]
+        L.omc_dev_poll.argtypes = [C.c_void_p, C.c_int, C.POINTER(Result), _dp, _dp]
         _lib = L
     return _lib
 
@@ -78,10 +90,11 @@ def check(rc):
 class Engine:
     """One device context: tables + resident phantoms."""
 
-    def __init__(self, precision=FP64, device=0, hbm_budget=0, chunk=0):
+    def __init__(self, precision=FP64, device=0, hbm_budget=0, chunk=0, pipelines=0):
         L = lib()
         self.ctx = C.c_void_p()
-        opts = DevOpts(device, precision, hbm_budget, chunk)
+        self.nreg = {}
+        opts = DevOpts(device, precision, hbm_budget, chunk, pipelines)
         check(L.omc_dev_create(C.byref(self.ctx), C.byref(opts)))
         self.precision = precision
 
@@ -102,6 +115,7 @@ class Engine:
     def upload_phantom(self, phantom, regions) -> int:
         pid = C.c_int(-1)
         check(lib().omc_dev_upload_phantom(self.ctx, C.byref(phantom), C.byref(regions), C.byref(pid)))
+        self.nreg[pid.value] = regions.nreg
         return pid.value
 
     def run_dosxyz(self, pid, src, nperbatch, nbatch, seed=(97, 33), hist0=0, nreg=None, ledger=False):
@@ -114,3 +128,60 @@ class Engine:
         check(lib().omc_dev_run_dosxyz(self.ctx, C.byref(src), C.byref(rp), s.ctypes.data_as(_dp),
                                        s2.ctypes.data_as(_dp)))
         return s, s2, stats, led
+
+    def run_beamlets(self, pid, src, nhist, nbatch, seed=(97, 33), hist0=0):
+        """Batch sums of every beamlet: arrays (nbeamlets, nreg)."""
+        nreg = self.nreg[pid]
+        nh = np.ascontiguousarray(nhist, dtype=np.int64)
+        stats = DevStats()
+        rp = RunParams(pid, 1, (C.c_uint32 * 2)(*seed), hist0, 0, nbatch, None, C.pointer(stats))
+        s = np.zeros((src.nbeamlets, nreg))
+        s2 = np.zeros((src.nbeamlets, nreg))
+        check(lib().omc_dev_run_beamlets(self.ctx, C.byref(src), nh.ctypes.data_as(C.POINTER(C.c_int64)),
+                                         C.byref(rp), s.ctypes.data_as(_dp), s2.ctypes.data_as(_dp)))
+        return s, s2, stats
+
+    def beamlets_sparse(self, pid, src, nhist, nbatch, rel_thr, seed=(97, 33), hist0=0, cap=None):
+        """matRad Dij as CSC arrays (jcs, irs, vals) plus the run statistics."""
+        nh = np.ascontiguousarray(nhist, dtype=np.int64)
+        stats = DevStats()
+        rp = RunParams(pid, 1, (C.c_uint32 * 2)(*seed), hist0, 0, nbatch, None, C.pointer(stats))
+        P64 = C.POINTER(C.c_int64)
+        nb = src.nbeamlets
+        cap = int(cap) if cap is not None else max(1, (self.nreg[pid] - 1) * nb // 50)
+        first = True
+        while True:
+            jcs = np.zeros(nb + 1, np.int64)
+            irs = np.zeros(cap, np.int64)
+            vals = np.zeros(cap)
+            nnz = C.c_int64(0)
+            rc = lib().omc_dev_beamlets_sparse(self.ctx, C.byref(src) if first else None,
+                                               nh.ctypes.data_as(P64), C.byref(rp), rel_thr,
+                                               jcs.ctypes.data_as(P64), irs.ctypes.data_as(P64),
+                                               vals.ctypes.data_as(_dp), cap, C.byref(nnz))
+            if rc == -6 and nnz.value > cap:          # OMC_ERANGE: collect the kept matrix
+                cap = nnz.value
+                first = False
+                continue
+            check(rc)
+            return jcs, irs[:nnz.value], vals[:nnz.value], stats
+
+    # ---- serving -------------------------------------------------------
+    def submit(self, pid, src, nhist, nbatch=10, seed=(97, 33), kind=0, beamlet=0):
+        req = Request(pid, kind, C.pointer(src) if kind == 0 else None, C.pointer(src) if kind == 1 else None,
+                      beamlet, int(nhist), nbatch, (C.c_uint32 * 2)(*seed))
+        return check(lib().omc_dev_submit(self.ctx, C.byref(req)))
+
+    def step(self):
+        return check(lib().omc_dev_step(self.ctx))
+
+    def poll(self, ticket, nreg=None):
+        """(Result, sum, sum2) -- sums are None until the ticket is done."""
+        res = Result()
+        if nreg is None:
+            check(lib().omc_dev_poll(self.ctx, ticket, C.byref(res), None, None))
+            return res, None, None
+        s = np.zeros(nreg)
+        s2 = np.zeros(nreg)
+        check(lib().omc_dev_poll(self.ctx, ticket, C.byref(res), s.ctypes.data_as(_dp), s2.ctypes.data_as(_dp)))
+        return (res, s, s2) if res.done and res.status == 0 else (res, None, None)

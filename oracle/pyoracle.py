@@ -13,6 +13,7 @@ from pathlib import Path
 import numpy as np
 
 from ompmc_amd import _native as N
+from ompmc_amd.problem import DosxyzProblem, free_problem, load_problem  # noqa: F401  (host loader)
 
 HERE = Path(__file__).resolve().parent
 LIB = HERE / "build" / "liboracle.so"
@@ -35,13 +36,6 @@ class Particle(C.Structure):
 class Counters(C.Structure):
     _fields_ = [("n_hist", C.c_int64), ("n_dep", C.c_int64), ("n_cross_photon", C.c_int64),
                 ("n_cross_charged", C.c_int64), ("n_draws", C.c_int64)]
-
-
-class DosxyzProblem(C.Structure):
-    _fields_ = [("inp_n", C.c_int), ("inp_key", (C.c_char * 256) * 80), ("inp_value", (C.c_char * 256) * 80),
-                ("ph", N.Phantom), ("tab", N.Tables), ("reg", N.Regions), ("src", N.SourceDosxyz),
-                ("nsplit", C.c_int), ("seeds", C.c_int * 2), ("nhist", C.c_int), ("nbatch", C.c_int),
-                ("nperbatch", C.c_int), ("output_folder", C.c_char * 256)]
 
 
 _lib = None
@@ -89,20 +83,6 @@ def lib():
         L.ora_philox4x32.argtypes = [C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]
         _lib = L
     return _lib
-
-
-def load_problem(stem) -> DosxyzProblem:
-    h = N.host()
-    h.omc_dosxyz_problem_load.argtypes = [C.POINTER(DosxyzProblem), C.c_char_p]
-    p = DosxyzProblem()
-    N.check(h.omc_dosxyz_problem_load(C.byref(p), str(stem).encode()))
-    return p
-
-
-def free_problem(p: DosxyzProblem):
-    h = N.host()
-    h.omc_dosxyz_problem_free.argtypes = [C.POINTER(DosxyzProblem)]
-    h.omc_dosxyz_problem_free(C.byref(p))
 
 
 def run_dosxyz_ranmar(prob: DosxyzProblem, nperbatch=None, nbatch=None):
@@ -170,3 +150,47 @@ def run_reference(stem, out_name, omp=False, threads=None, timeout=600):
     if r.returncode != 0:
         raise RuntimeError(f"reference failed: {r.stdout[-2000:]} {r.stderr[-2000:]}")
     return r.stdout + r.stderr
+
+
+def run_beamlet_philox(prob: DosxyzProblem, src, beamlet, nperbatch, nbatch, seed=(97, 33), hist0=0):
+    """matRad beamlet loop for one beamlet (omc_matrad.c:1389-1414) in
+    PHILOX mode: histories hist0 .. hist0 + nperbatch*nbatch - 1; returns the
+    batch sums (sum, sum2) over all regions."""
+    L = lib()
+    ctx = L.ora_create(C.byref(prob.tab), C.byref(prob.reg), C.byref(prob.ph), 1, PHILOX)
+    try:
+        L.ora_seed_philox(ctx, seed[0], seed[1])
+        nreg = prob.reg.nreg
+        s = np.zeros(nreg)
+        s2 = np.zeros(nreg)
+        rc = L.ora_run_beamlet(ctx, C.byref(src), beamlet, hist0, nperbatch, nbatch, s.ctypes.data_as(_dp),
+                               s2.ctypes.data_as(_dp))
+        if rc:
+            raise RuntimeError(f"oracle beamlet run failed {rc}")
+        return s, s2
+    finally:
+        L.ora_destroy(ctx)
+
+
+def matrad_column(xb, yb, zb, rho, s, nbatch, nhist, rel_thr):
+    """One column of the matRad dose influence matrix from a beamlet's batch
+    sums `s` ([nvox+1], region 0 first): accumulateResults with iout = 1
+    (omc_matrad.c:839-918: mean over batches, Gy per incident history,
+    zero below density 0.044), then the threshold at rel_thr * max and the
+    rows in voxel order (omc_matrad.c:1420-1479).  Same operation order as
+    the reference, element by element.  Returns (rows, values)."""
+    xb, yb, zb = (np.asarray(a, dtype=np.float64) for a in (xb, yb, zb))
+    ni, nj, nk = len(xb) - 1, len(yb) - 1, len(zb) - 1
+    rho = np.asarray(rho, dtype=np.float64).reshape(nk, nj, ni)
+    e = np.asarray(s, dtype=np.float64)[1:].reshape(nk, nj, ni) / float(nbatch)
+    dx, dy, dz = np.diff(xb), np.diff(yb), np.diff(zb)
+    mass = (dx[None, None, :] * dy[None, :, None]) * dz[:, None, None]
+    mass = mass * rho
+    e = e * (1.602E-10 / (mass * float(nhist)))
+    e = np.where(rho < 0.044, 0.0, e).reshape(-1)
+    dmax = 0.0
+    if e.size and e.max() > dmax:
+        dmax = float(e.max())
+    thresh = dmax * rel_thr
+    rows = np.nonzero(e > thresh)[0].astype(np.int64)
+    return rows, e[rows]

@@ -97,10 +97,12 @@ def _mv(s, s2, nb):
 
 @pytest.mark.parametrize("case", ["slab6MV", "patient6MV", "electron6MeV"])
 def test_fp32_agrees_with_fp64_within_2_sigma(oracle, D, tmp_path, case):
-    """FP32 transport with FP64 tallies against the FP64 kernel (different
-    rounding, so independent histories after the first divergence): >= 95%
-    of voxels above 10% Dmax within 2 combined sigma (50 batches), integral
-    energy within 0.5%."""
+    """FP32 transport with FP64 tallies against the FP64 kernel: >= 95% of
+    voxels above 10% Dmax within 2 combined sigma (50 batches), integral
+    energy within 0.5%.  Both precisions draw from the same per-particle
+    streams (same seed); a history diverges only once rounding changes a
+    branch, so the integral comparison measures precision, not the 0.5%
+    relative standard deviation of one 200k-history photon run."""
     stem = write_case(case, tmp_path, ncase=200000)
     inp = tmp_path / f"{case}.inp"
     inp.write_text(inp.read_text().replace("nbatch = 10", "nbatch = 50"))
@@ -111,7 +113,7 @@ def test_fp32_agrees_with_fp64_within_2_sigma(oracle, D, tmp_path, case):
             with D.Engine(prec) as eng:
                 eng.upload_media(prob.tab)
                 pid = eng.upload_phantom(prob.ph, prob.reg)
-                res[prec] = eng.run_dosxyz(pid, prob.src, prob.nperbatch, prob.nbatch, seed=(8, 1 + prec),
+                res[prec] = eng.run_dosxyz(pid, prob.src, prob.nperbatch, prob.nbatch, seed=(8, 1),
                                            nreg=prob.reg.nreg)
         nb = prob.nbatch
     finally:
@@ -146,42 +148,48 @@ def test_device_matches_reference_rng_statistically(oracle, D, tmp_path):
     assert abs(d[1:].sum() / r[1:].sum() - 1) < 0.01
 
 
-def test_chunking_and_reruns_do_not_change_histories(oracle, D, tmp_path):
-    """Histories are identified by (seed, index) only: splitting the run into
-    small chunks, or repeating it, reproduces the same energies up to FP64
+def test_schedule_and_reruns_do_not_change_histories(oracle, D, tmp_path):
+    """Histories are identified by (seed, index) only: a different number of
+    pipelines, a small refill chunk (many injections, primaries mixed with
+    older secondaries) or a repeat reproduce the same energies up to FP64
     summation order."""
     stem = write_case("patient6MV", tmp_path, ncase=5000)
     prob = oracle.load_problem(stem)
     try:
         out = []
-        for chunk in (0, 0, 777):
-            with D.Engine(0, chunk=chunk) as eng:
+        for chunk, pipes in ((0, 0), (0, 1), (777, 3)):
+            with D.Engine(0, chunk=chunk, pipelines=pipes) as eng:
                 eng.upload_media(prob.tab)
                 pid = eng.upload_phantom(prob.ph, prob.reg)
                 s, _, st, led = eng.run_dosxyz(pid, prob.src, prob.nperbatch, prob.nbatch, seed=(4, 4),
                                                nreg=prob.reg.nreg, ledger=True)
-                out.append((s, led, st.chunks))
+                out.append((s, led, st.injections))
     finally:
         oracle.free_problem(prob)
-    assert out[2][2] > 1
+    assert out[2][2] > 5
     for s, led, _ in out[1:]:
         assert np.allclose(s, out[0][0], rtol=1e-12, atol=1e-12 * out[0][0].max())
         assert np.allclose(led, out[0][1], rtol=1e-12, atol=1e-12)
 
 
-def test_vacuum_voxels_inside_phantom(oracle, D, tmp_path):
+@pytest.mark.parametrize("energy,charge", [(3.0, 0), (6.0, -1), (4.0, 1)])
+def test_vacuum_voxels_inside_phantom(oracle, D, tmp_path, energy, charge):
     """Medium 0 voxels (vacuum) inside the grid: straight-line transport,
-    same histories as the oracle."""
+    same histories as the oracle, including charged particles that cross
+    the gap and re-enter the medium they left (where the reference goes on
+    to the fictitious-sigma test without restarting the step)."""
     from ompmc_amd import configs as cfg
     ph = cfg.water_cube(10, 5.0)
     ph.med[4:6, 3:7, 3:7] = 0
     ph.rho[4:6, 3:7, 3:7] = 0.0
     pf = ph.write_egsphant(tmp_path / "vac.egsphant")
-    items = cfg.dosxyz_inp_items(tmp_path, pf, ncase=1000, mono=3.0, field=(-1, 1, -1, 1))
+    items = cfg.dosxyz_inp_items(tmp_path, pf, ncase=1000, mono=energy, charge=charge, field=(-1, 1, -1, 1))
     cfg.write_inp(tmp_path / "vac", items)
     r = run_both(oracle, D, tmp_path / "vac")
     scale = np.abs(r["so"]).max()
     assert (np.abs(r["sd"] - r["so"]) <= 1e-9 * scale).mean() >= 0.999
+    same = np.all(np.abs(r["led_d"] - r["led_o"]) <= 1e-9 * (1 + np.abs(r["led_o"])), axis=1)
+    assert same.mean() >= 0.995, f"{np.count_nonzero(~same)} histories differ"
 
 
 def test_error_paths_return_codes(oracle, D, tmp_path):
@@ -225,3 +233,25 @@ def test_single_history_and_single_batch(oracle, D, tmp_path):
         oracle.free_problem(prob)
     assert st.histories == 1
     assert np.allclose(sd, so, rtol=1e-12, atol=1e-12) and np.allclose(sd2, sd * sd, rtol=1e-12, atol=1e-15)
+
+
+@pytest.mark.parametrize("case", ["slab6MV", "electron6MeV"])
+def test_device_counters_equal_oracle_counts(oracle, D, tmp_path, case):
+    """FP64 replay: the device's per-kernel counters of ausgab calls and voxel
+    crossings (the units of the algorithmic-bytes roofline) equal the
+    oracle's counts for the same histories."""
+    stem = write_case(case, tmp_path, ncase=2000)
+    prob = oracle.load_problem(stem)
+    try:
+        _, _, cnt = oracle.run_dosxyz_philox(prob, seed=(3, 4), nthreads=1)
+        with D.Engine(0) as eng:
+            eng.upload_media(prob.tab)
+            pid = eng.upload_phantom(prob.ph, prob.reg)
+            _, _, st, _ = eng.run_dosxyz(pid, prob.src, prob.nperbatch, prob.nbatch, seed=(3, 4),
+                                         nreg=prob.reg.nreg)
+    finally:
+        oracle.free_problem(prob)
+    assert st.deposits == st.photon_deposits + st.electron_deposits
+    for dev, ora in ((st.deposits, cnt.n_dep), (st.photon_crossings, cnt.n_cross_photon),
+                     (st.electron_crossings, cnt.n_cross_charged)):
+        assert abs(dev - ora) <= 1e-3 * ora + 2, (dev, ora)
