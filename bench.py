@@ -291,10 +291,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--hist", type=int, default=4_000_000)
+    ap.add_argument("--hist", type=int, default=20_000_000)
     ap.add_argument("--pipelines", type=int, default=0)
     ap.add_argument("--chunk", type=int, default=0)
-    ap.add_argument("--cpu-sample", type=int, default=300_000)
+    ap.add_argument("--cpu-sample", type=int, default=10_000_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-beamlets", action="store_true")

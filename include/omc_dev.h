@@ -44,6 +44,10 @@ typedef struct {
                                 are held back; 0 = automatic */
     int     pipelines;       /* concurrent transport pipelines (streams);
                                 0 = automatic */
+    int     yield_trips;     /* transport loop trips a particle may take in
+                                one launch before it continues in the next
+                                (its lane state travels with it, so results
+                                do not depend on this); 0 = automatic */
 } omc_dev_opts;
 
 /* Device counters of one run (all sums over the run). */
@@ -63,6 +67,7 @@ typedef struct {
     int64_t killed;             /* particles stopped by the step-limit watchdog */
     int64_t photon_crossings, electron_crossings;   /* voxel boundaries crossed */
     int64_t photon_deposits, electron_deposits;     /* ausgab calls per kernel */
+    int64_t yields;             /* particles handed on to a later launch */
 } omc_dev_stats;
 
 /* Context: device, pipelines (each a stream with its own particle queues),

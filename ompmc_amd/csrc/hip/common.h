@@ -196,16 +196,23 @@ struct SourceSeg {
     uint32_t job, n, off, pad;
 };
 
-// Structure-of-arrays particle queue.
+// Structure-of-arrays particle queue.  A particle that gave up its lane in
+// the middle of its transport (a yield) also carries the lane's state, so
+// the next launch resumes it exactly where it stopped.
 template <typename R> struct Queue {
     R *e, *x, *y, *z, *u, *v, *w, *wt;
     int32_t *ir;
-    uint32_t *meta;          // job << 2 | (iq + 1)
+    uint32_t *meta;          // job << 3 | resume << 2 | (iq + 1)
     uint32_t *hist;          // history index within the job
     uint32_t *k0, *k1, *ctr;
+    R *st[6];                // resume state: real-valued lane variables
+    uint32_t *sflags;        // resume state: packed integer lane variables
+    uint32_t *trips;         // transport loop trips taken so far
     uint32_t *count;         // device tail counter
     uint32_t cap;
 };
+
+constexpr uint32_t META_RESUME = 4u;
 
 template <typename R> struct Particle {
     R e, x, y, z, u, v, w, wt;
@@ -214,27 +221,29 @@ template <typename R> struct Particle {
     Rng rng;
 };
 
+// Returns true if the entry carries a resume state.
 template <typename R>
-__device__ __forceinline__ void q_load(const Queue<R> &q, uint32_t i, Particle<R> &p)
+__device__ __forceinline__ bool q_load(const Queue<R> &q, uint32_t i, Particle<R> &p)
 {
     p.e = q.e[i]; p.x = q.x[i]; p.y = q.y[i]; p.z = q.z[i];
     p.u = q.u[i]; p.v = q.v[i]; p.w = q.w[i]; p.wt = q.wt[i];
     p.ir = q.ir[i];
     uint32_t m = q.meta[i];
-    p.job = m >> 2;
+    p.job = m >> 3;
     p.iq = (int)(m & 3u) - 1;
     p.hist = q.hist[i];
     p.rng.k0 = q.k0[i]; p.rng.k1 = q.k1[i]; p.rng.ctr = q.ctr[i];
     p.rng.reset_cache();
+    return (m & META_RESUME) != 0u;
 }
 
 template <typename R>
-__device__ __forceinline__ void q_store(const Queue<R> &q, uint32_t i, const Particle<R> &p)
+__device__ __forceinline__ void q_store(const Queue<R> &q, uint32_t i, const Particle<R> &p, uint32_t resume = 0u)
 {
     q.e[i] = p.e; q.x[i] = p.x; q.y[i] = p.y; q.z[i] = p.z;
     q.u[i] = p.u; q.v[i] = p.v; q.w[i] = p.w; q.wt[i] = p.wt;
     q.ir[i] = p.ir;
-    q.meta[i] = (p.job << 2) | (uint32_t)(p.iq + 1);
+    q.meta[i] = (p.job << 3) | resume | (uint32_t)(p.iq + 1);
     q.hist[i] = p.hist;
     q.k0[i] = p.rng.k0; q.k1[i] = p.rng.k1; q.ctr[i] = p.rng.ctr;
 }
@@ -243,6 +252,7 @@ __device__ __forceinline__ void q_store(const Queue<R> &q, uint32_t i, const Par
 struct DevStatus {
     uint32_t overflow;       // a push found its queue full
     uint32_t killed;         // particles stopped by the per-particle step limit
+    unsigned long long yields;                         // particles handed to the next launch
     unsigned long long photon_steps, electron_steps;   // transport loop trips
     unsigned long long photon_cross, electron_cross;   // voxel boundaries crossed
     unsigned long long photon_dep, electron_dep;       // ausgab calls

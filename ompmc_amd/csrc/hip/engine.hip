@@ -65,7 +65,12 @@ constexpr int MAX_JOBS = 4096;
 constexpr int MAX_SEGS = 512;
 constexpr int MAX_PIPES = 8;
 constexpr int DEFAULT_PIPES = 4;
-constexpr int64_t DEFAULT_CHUNK = 1 << 18;
+// Defaults from a sweep of pipelines x chunk x yield budget on the 64^3 slab
+// phantom with the 6 MV spectrum (16M histories, FP32 and FP64): throughput
+// grows with the particles per launch (lanes then refill many times before
+// the launch drains) and with overlapping pipelines.
+constexpr int64_t DEFAULT_CHUNK = 1 << 21;
+constexpr uint32_t DEFAULT_YIELD_TRIPS = 4096;
 
 struct PhantomSlot {
     bool used = false;
@@ -85,7 +90,7 @@ struct PipeMirror {
 
 struct Pipe {
     hipStream_t stream = nullptr;
-    void *q[4][13] = {};          // [photon cur/next, charged cur/next][field]
+    void *q[4][21] = {};          // [photon cur/next, charged cur/next][field]
     uint32_t *d_counts = nullptr; // 4 tails + 2 fetch counters
     DevStatus *d_status = nullptr;
     uint32_t *d_flags = nullptr;  // [MAX_JOBS]
@@ -162,6 +167,7 @@ struct omc_dev_ctx {
     int64_t used = 0;
     int64_t chunk = 0;
     uint32_t qcap = 0;
+    uint32_t yield_trips = 0;
     int cus = 256;
     std::chrono::steady_clock::time_point t0;
 
@@ -245,7 +251,8 @@ void pool_free(omc_dev_ctx *c, void *p, size_t bytes)
     if (p) c->pool.emplace(bytes, p);
 }
 
-template <typename R> size_t particle_bytes() { return 8 * sizeof(R) + 6 * sizeof(uint32_t); }
+// per queue slot: 8 reals, ir/meta/hist/k0, k1+ctr, 6 resume reals, sflags, trips
+template <typename R> size_t particle_bytes() { return 14 * sizeof(R) + 8 * sizeof(uint32_t); }
 
 template <typename R> Queue<R> queue_view(const omc_dev_ctx *c, const Pipe &P, int k)
 {
@@ -256,6 +263,9 @@ template <typename R> Queue<R> queue_view(const omc_dev_ctx *c, const Pipe &P, i
     q.ir = (int32_t *)a[8]; q.meta = (uint32_t *)a[9]; q.hist = (uint32_t *)a[10];
     q.k0 = (uint32_t *)a[11]; q.k1 = (uint32_t *)a[12];
     q.ctr = (uint32_t *)a[12] + c->qcap;     // k1 and ctr share one allocation
+    for (int f = 0; f < 6; ++f) q.st[f] = (R *)a[13 + f];
+    q.sflags = (uint32_t *)a[19];
+    q.trips = (uint32_t *)a[20];
     q.count = P.d_counts + k;
     q.cap = c->qcap;
     return q;
@@ -270,6 +280,8 @@ template <typename R> int alloc_pipe(omc_dev_ctx *c, Pipe &P)
         for (int f = 0; f < 8; ++f) OMC_TRY_DEV(dalloc(c, &P.q[k][f], sizeof(R) * (size_t)c->qcap));
         for (int f = 8; f < 12; ++f) OMC_TRY_DEV(dalloc(c, &P.q[k][f], 4 * (size_t)c->qcap));
         OMC_TRY_DEV(dalloc(c, &P.q[k][12], 8 * (size_t)c->qcap));
+        for (int f = 13; f < 19; ++f) OMC_TRY_DEV(dalloc(c, &P.q[k][f], sizeof(R) * (size_t)c->qcap));
+        for (int f = 19; f < 21; ++f) OMC_TRY_DEV(dalloc(c, &P.q[k][f], 4 * (size_t)c->qcap));
     }
     OMC_TRY_DEV(dalloc(c, (void **)&P.d_counts, 8 * sizeof(uint32_t)));
     OMC_TRY_DEV(dalloc(c, (void **)&P.d_status, sizeof(DevStatus)));
@@ -550,7 +562,11 @@ static int admit(omc_dev_ctx *c)
         }
         J.used = true;
         J.w = w;
-        J.quantum = w->ticket >= 0 ? std::max<int64_t>(4096, c->chunk / 8) : INT64_MAX;
+        // serving requests share each refill; a synchronous run spreads its
+        // histories over all pipelines
+        const int64_t npipes = (int64_t)c->pipes.size();
+        J.quantum = w->ticket >= 0 ? std::max<int64_t>(4096, c->chunk / 8)
+                                   : std::max<int64_t>(65536, (w->nhist + npipes - 1) / npipes);
         launch_zero_f64(J.tally, w->nreg * w->nbatch, c->aux);
         if (J.ledger) launch_zero_f64(J.ledger, 3 * w->nhist, c->aux);
         HIPCHK(hipMemcpyAsync(J.cdf, w->cdf.data(), 2 * OMC_INVDIM * sizeof(double), hipMemcpyHostToDevice, c->aux));
@@ -690,6 +706,7 @@ template <typename R> static int launch_generation(omc_dev_ctx *c, int pi)
     A.qp = queue_view<R>(c, P, nxt);
     A.qe = queue_view<R>(c, P, 2 + nxt);
     A.status = P.d_status;
+    A.yield_trips = c->yield_trips;
     const int max_grid = c->cus * 8;
     P.had_ph = inP > 0;
     if (P.had_ph) {
@@ -865,6 +882,7 @@ static void stats_end(omc_dev_ctx *c, omc_dev_stats *out, int64_t histories, dou
         S.photon_deposits += (int64_t)(P.h->status.photon_dep - P.base.photon_dep);
         S.electron_deposits += (int64_t)(P.h->status.electron_dep - P.base.electron_dep);
         S.killed += (int64_t)(P.h->status.killed - P.base.killed);
+        S.yields += (int64_t)(P.h->status.yields - P.base.yields);
     }
     S.deposits = S.photon_deposits + S.electron_deposits;
     S.histories = histories;
@@ -909,12 +927,13 @@ int omc_dev_create(omc_dev_ctx **out, const omc_dev_opts *opts)
 {
     if (!out) return fail(OMC_EARG, "null context pointer");
     *out = nullptr;
-    omc_dev_opts o = { 0, OMC_PREC_FP64, 0, 0, 0 };
+    omc_dev_opts o = { 0, OMC_PREC_FP64, 0, 0, 0, 0 };
     if (opts) o = *opts;
     if (o.precision != OMC_PREC_FP64 && o.precision != OMC_PREC_FP32)
         return fail(OMC_EARG, "unknown precision %d", o.precision);
     if (o.pipelines < 0 || o.pipelines > MAX_PIPES) return fail(OMC_ERANGE, "pipelines %d (1..%d)", o.pipelines, MAX_PIPES);
     if (o.chunk < 0) return fail(OMC_EARG, "chunk %lld", (long long)o.chunk);
+    if (o.yield_trips < 0) return fail(OMC_EARG, "yield_trips %d", o.yield_trips);
     int ndev = 0;
     HIPCHK(hipGetDeviceCount(&ndev));
     if (o.device < 0 || o.device >= ndev) return fail(OMC_EDEVICE, "device %d not present (%d devices)", o.device, ndev);
@@ -923,6 +942,7 @@ int omc_dev_create(omc_dev_ctx **out, const omc_dev_opts *opts)
     c->t0 = std::chrono::steady_clock::now();
     c->device = o.device;
     c->precision = o.precision;
+    c->yield_trips = o.yield_trips > 0 ? (uint32_t)o.yield_trips : DEFAULT_YIELD_TRIPS;
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, o.device) == hipSuccess) c->cus = prop.multiProcessorCount;
     size_t freeb = 0, totalb = 0;

@@ -54,7 +54,7 @@ __device__ __forceinline__ void flag_jobs(const uint32_t *meta, uint32_t n, uint
     for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
         uint32_t i = base + threadIdx.x;
         bool have = i < n;
-        uint32_t job = have ? (meta[i] >> 2) : 0u;
+        uint32_t job = have ? (meta[i] >> 3) : 0u;
         while (__ballot(have)) {
             uint32_t lead = __shfl(job, __ffsll((unsigned long long)__ballot(have)) - 1);
             bool mine = have && job == lead;

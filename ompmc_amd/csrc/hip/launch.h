@@ -27,7 +27,7 @@ void launch_zero_f64(double *p, int64_t n, hipStream_t st);
 void launch_gen_reset(uint32_t *counts, int nxt, hipStream_t st);
 
 // flags[j] = 1 for every job j that owns a particle in either queue
-// (meta[] >> 2), over the first min(*count, cap) entries; flags must be
+// (meta[] >> 3), over the first min(*count, cap) entries; flags must be
 // cleared beforehand.
 void launch_job_flags(const uint32_t *meta_p, const uint32_t *count_p, const uint32_t *meta_e,
                       const uint32_t *count_e, uint32_t cap, uint32_t *flags, int njobs, hipStream_t st);

@@ -17,8 +17,8 @@
 namespace omc {
 
 constexpr double SGMFP = 1.0E-05;
-// Loop trips one particle may take within one launch before the lane gives
-// it up (counted in DevStatus::killed); far above any physical history.
+// Loop trips one particle may take over all launches before it is given up
+// (counted in DevStatus::killed); far above any physical history.
 constexpr uint32_t MAX_ITERS = 1u << 22;
 
 template <typename R> struct KArgs {
@@ -31,6 +31,7 @@ template <typename R> struct KArgs {
     Queue<R> qp;             // photon output queue
     Queue<R> qe;             // charged-particle output queue
     DevStatus *status;
+    uint32_t yield_trips;    // loop trips a particle may take per launch
 };
 
 // Per-lane geometry / scoring context for the particle the lane holds.
@@ -92,11 +93,56 @@ __device__ __forceinline__ void push(const Queue<R> &q, const Particle<R> &p, De
     else atomicOr(&st->overflow, 1u);
 }
 
+// Packing of a lane's integer variables into one word of the resume state:
+// bits 0-1 phase, 2 compute_tstep, 3-6 medium + 1, 7-10 previous medium + 1,
+// 11-22 table bin + 1.
+__device__ __forceinline__ uint32_t pack_lane(int phase, int flag, int imed, int medold, int bin)
+{
+    return (uint32_t)phase | ((uint32_t)flag << 2) | ((uint32_t)(imed + 1) << 3) | ((uint32_t)(medold + 1) << 7) |
+           ((uint32_t)(bin + 1) << 11);
+}
+__device__ __forceinline__ void unpack_lane(uint32_t w, int &phase, int &flag, int &imed, int &medold, int &bin)
+{
+    phase = (int)(w & 3u);
+    flag = (int)((w >> 2) & 1u);
+    imed = (int)((w >> 3) & 15u) - 1;
+    medold = (int)((w >> 7) & 15u) - 1;
+    bin = (int)((w >> 11) & 4095u) - 1;
+}
+
+// Hands a particle that is still in flight to the next launch together with
+// the state its lane needs to continue (see PhotonLane / ElectronLane).
+template <typename R, typename Lane>
+__device__ __forceinline__ void yield_push(const Queue<R> &q, const Particle<R> &p, const Lane &s, uint32_t trips,
+                                           DevStatus *st)
+{
+    uint32_t i = wave_reserve(q.count, 1);
+    if (i < q.cap) {
+        q_store(q, i, p, META_RESUME);
+        s.save(q, i);
+        q.trips[i] = trips;
+    } else {
+        atomicOr(&st->overflow, 1u);
+    }
+}
+
 // ============================================================ photons ======
 template <typename R> struct PhotonLane {
     int phase;               // 0: (re)enter photon(), 1: transport step
     int imed, lgle;
     R gle, dpmfp, gmfp, cohfac, eig;
+
+    __device__ __forceinline__ void save(const Queue<R> &q, uint32_t i) const
+    {
+        q.st[0][i] = gle; q.st[1][i] = dpmfp; q.st[2][i] = gmfp; q.st[3][i] = cohfac; q.st[4][i] = eig;
+        q.sflags[i] = pack_lane(phase, 0, imed, -1, lgle);
+    }
+    __device__ __forceinline__ void load(const Queue<R> &q, uint32_t i)
+    {
+        gle = q.st[0][i]; dpmfp = q.st[1][i]; gmfp = q.st[2][i]; cohfac = q.st[3][i]; eig = q.st[4][i];
+        int flag, medold;
+        unpack_lane(q.sflags[i], phase, flag, imed, medold, lgle);
+    }
 };
 
 // Returns true while the photon is still owned by the lane.  Electrons it
@@ -195,6 +241,19 @@ template <typename R> struct ElectronLane {
     int phase;               // 0 entry, 1 head of tstep loop, 2 ustep iteration
     int imed, medold, lelke, compute_tstep;
     R rhof, eke, elke, demfp, sig0, ustep;
+
+    __device__ __forceinline__ void save(const Queue<R> &q, uint32_t i) const
+    {
+        q.st[0][i] = rhof; q.st[1][i] = eke; q.st[2][i] = elke;
+        q.st[3][i] = demfp; q.st[4][i] = sig0; q.st[5][i] = ustep;
+        q.sflags[i] = pack_lane(phase, compute_tstep, imed, medold, lelke);
+    }
+    __device__ __forceinline__ void load(const Queue<R> &q, uint32_t i)
+    {
+        rhof = q.st[0][i]; eke = q.st[1][i]; elke = q.st[2][i];
+        demfp = q.st[3][i]; sig0 = q.st[4][i]; ustep = q.st[5][i];
+        unpack_lane(q.sflags[i], phase, compute_tstep, imed, medold, lelke);
+    }
 };
 
 enum { E_ALIVE = 0, E_DONE = 1 };
@@ -549,15 +608,21 @@ __global__ void __launch_bounds__(256) k_photon(KArgs<R> A)
     LaneGeo<R> G;
     Scorer<R> sc;
     bool alive = false, exhausted = false;
-    uint32_t nsteps = 0, ncross = 0, iters = 0;
+    uint32_t nsteps = 0, ncross = 0, trips = 0, here = 0, nyield = 0;
     const uint32_t in_end = min(*A.in_count, A.qin.cap);
     for (;;) {
         if (!alive && !exhausted) {
             uint32_t i = wave_reserve(A.fetch, 1);
             if (i < in_end) {
-                q_load(A.qin, i, p);
+                if (q_load(A.qin, i, p)) {
+                    s.load(A.qin, i);
+                    trips = A.qin.trips[i];
+                } else {
+                    s.phase = 0;
+                    trips = 0;
+                }
                 G.bind(A, p);
-                s.phase = 0;
+                here = 0;
                 alive = true;
             } else {
                 exhausted = true;
@@ -566,14 +631,21 @@ __global__ void __launch_bounds__(256) k_photon(KArgs<R> A)
         if (__ballot(alive) == 0) break;
         if (alive) {
             alive = photon_iteration<R>(A, p, s, G, sc, nsteps, ncross);
-            if (alive && ++iters > MAX_ITERS) {         // watchdog: no lane runs forever
+            ++trips;
+            if (alive && trips > MAX_ITERS) {           // watchdog: no particle runs forever
                 atomicAdd(&A.status->killed, 1u);
                 alive = false;
+            } else if (alive && ++here >= A.yield_trips) {
+                sc.flush(G);
+                yield_push<R>(A.qp, p, s, trips, A.status);
+                nyield++;
+                alive = false;
             }
-            if (!alive) { sc.flush(G); iters = 0; }
+            if (!alive) sc.flush(G);
         }
     }
     sc.flush(G);
+    if (nyield) atomicAdd(&A.status->yields, (unsigned long long)nyield);
     atomicAdd(&A.status->photon_steps, (unsigned long long)nsteps);
     atomicAdd(&A.status->photon_cross, (unsigned long long)ncross);
     atomicAdd(&A.status->photon_dep, (unsigned long long)sc.ndep);
@@ -587,15 +659,21 @@ __global__ void __launch_bounds__(256) k_electron(KArgs<R> A)
     LaneGeo<R> G;
     Scorer<R> sc;
     bool alive = false, exhausted = false;
-    uint32_t nsteps = 0, ncross = 0, iters = 0;
+    uint32_t nsteps = 0, ncross = 0, trips = 0, here = 0, nyield = 0;
     const uint32_t in_end = min(*A.in_count, A.qin.cap);
     for (;;) {
         if (!alive && !exhausted) {
             uint32_t i = wave_reserve(A.fetch, 1);
             if (i < in_end) {
-                q_load(A.qin, i, p);
+                if (q_load(A.qin, i, p)) {
+                    s.load(A.qin, i);
+                    trips = A.qin.trips[i];
+                } else {
+                    s.phase = 0;
+                    trips = 0;
+                }
                 G.bind(A, p);
-                s.phase = 0;
+                here = 0;
                 alive = true;
             } else {
                 exhausted = true;
@@ -604,18 +682,24 @@ __global__ void __launch_bounds__(256) k_electron(KArgs<R> A)
         if (__ballot(alive) == 0) break;
         if (alive) {
             int r = electron_iteration<R>(A, p, s, G, sc, nsteps, ncross);
-            if (r != E_DONE && ++iters > MAX_ITERS) {   // watchdog: no lane runs forever
+            ++trips;
+            if (r != E_DONE && trips > MAX_ITERS) {     // watchdog: no particle runs forever
                 atomicAdd(&A.status->killed, 1u);
+                r = E_DONE;
+            } else if (r != E_DONE && ++here >= A.yield_trips) {
+                sc.flush(G);
+                yield_push<R>(A.qe, p, s, trips, A.status);
+                nyield++;
                 r = E_DONE;
             }
             if (r == E_DONE) {
                 alive = false;
                 sc.flush(G);
-                iters = 0;
             }
         }
     }
     sc.flush(G);
+    if (nyield) atomicAdd(&A.status->yields, (unsigned long long)nyield);
     atomicAdd(&A.status->electron_steps, (unsigned long long)nsteps);
     atomicAdd(&A.status->electron_cross, (unsigned long long)ncross);
     atomicAdd(&A.status->electron_dep, (unsigned long long)sc.ndep);

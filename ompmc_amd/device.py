@@ -15,7 +15,7 @@ _dp = C.POINTER(C.c_double)
 
 class DevOpts(C.Structure):
     _fields_ = [("device", C.c_int), ("precision", C.c_int), ("hbm_budget", C.c_int64), ("chunk", C.c_int64),
-                ("pipelines", C.c_int)]
+                ("pipelines", C.c_int), ("yield_trips", C.c_int)]
 
 
 class DevStats(C.Structure):
@@ -26,7 +26,7 @@ class DevStats(C.Structure):
                 ("source_kernel_ms", C.c_double), ("photon_launches", C.c_int64),
                 ("electron_launches", C.c_int64), ("killed", C.c_int64), ("photon_crossings", C.c_int64),
                 ("electron_crossings", C.c_int64), ("photon_deposits", C.c_int64),
-                ("electron_deposits", C.c_int64)]
+                ("electron_deposits", C.c_int64), ("yields", C.c_int64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -90,11 +90,11 @@ def check(rc):
 class Engine:
     """One device context: tables + resident phantoms."""
 
-    def __init__(self, precision=FP64, device=0, hbm_budget=0, chunk=0, pipelines=0):
+    def __init__(self, precision=FP64, device=0, hbm_budget=0, chunk=0, pipelines=0, yield_trips=0):
         L = lib()
         self.ctx = C.c_void_p()
         self.nreg = {}
-        opts = DevOpts(device, precision, hbm_budget, chunk, pipelines)
+        opts = DevOpts(device, precision, hbm_budget, chunk, pipelines, yield_trips)
         check(L.omc_dev_create(C.byref(self.ctx), C.byref(opts)))
         self.precision = precision
 

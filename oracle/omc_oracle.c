@@ -1913,6 +1913,14 @@ int ora_run_beamlet(ora_ctx *c, const omc_source_beamlets *src, int ibeamlet, in
     return OMC_OK;
 }
 
+int ora_beamlet_primary(ora_ctx *c, const omc_source_beamlets *src, int ibeamlet, int64_t hid, ora_particle *out)
+{
+    if (ibeamlet < 0 || ibeamlet >= src->nbeamlets) return OMC_ERANGE;
+    init_history_beamlet(c, src, ibeamlet, hid);
+    *out = c->S[0];
+    return OMC_OK;
+}
+
 int ora_run_dosxyz_parallel(const omc_tables *t, const omc_regions *r, const omc_phantom *g,
                             const omc_source_dosxyz *src, uint32_t s0, uint32_t s1, int64_t hist0,
                             int nperbatch, int nbatch, int nthreads, double *sum, double *sum2,

@@ -86,6 +86,11 @@ int ora_run_dosxyz_parallel(const omc_tables *t, const omc_regions *r, const omc
 int ora_run_beamlet(ora_ctx *c, const omc_source_beamlets *src, int ibeamlet,
                     int64_t hist0, int nperbatch, int nbatch, double *sum, double *sum2);
 
+/* Primary particle of history `hid` of a beamlet (initHistory,
+ * omc_matrad.c:1084-1254). */
+int ora_beamlet_primary(ora_ctx *c, const omc_source_beamlets *src, int ibeamlet, int64_t hid,
+                        ora_particle *out);
+
 /* ---- per-function entry points for parity vectors ------------------------ */
 /* howfar/hownear on one particle state. */
 void   ora_howfar(const ora_ctx *c, const ora_particle *p, int *idisc, int *irnew, double *ustep);

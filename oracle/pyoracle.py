@@ -66,6 +66,8 @@ def lib():
                                               C.c_int, C.c_int, C.c_int, _dp, _dp, C.POINTER(Counters)]
         L.ora_run_beamlet.argtypes = [C.c_void_p, C.POINTER(N.SourceBeamlets), C.c_int, C.c_int64,
                                       C.c_int, C.c_int, _dp, _dp]
+        L.ora_beamlet_primary.argtypes = [C.c_void_p, C.POINTER(N.SourceBeamlets), C.c_int, C.c_int64,
+                                          C.POINTER(Particle)]
         L.ora_howfar.argtypes = [C.c_void_p, C.POINTER(Particle), C.POINTER(C.c_int), C.POINTER(C.c_int), _dp]
         L.ora_hownear.argtypes = [C.c_void_p, C.POINTER(Particle)]
         L.ora_hownear.restype = C.c_double
@@ -194,3 +196,20 @@ def matrad_column(xb, yb, zb, rho, s, nbatch, nhist, rel_thr):
     thresh = dmax * rel_thr
     rows = np.nonzero(e > thresh)[0].astype(np.int64)
     return rows, e[rows]
+
+
+def beamlet_primaries(prob: DosxyzProblem, src, beamlet, hids, seed=(97, 33)):
+    """Primary particles of the given histories of one beamlet (PHILOX mode)."""
+    L = lib()
+    ctx = L.ora_create(C.byref(prob.tab), C.byref(prob.reg), C.byref(prob.ph), 1, PHILOX)
+    try:
+        L.ora_seed_philox(ctx, seed[0], seed[1])
+        out = []
+        for h in hids:
+            p = Particle()
+            if L.ora_beamlet_primary(ctx, C.byref(src), beamlet, int(h), C.byref(p)):
+                raise RuntimeError("bad beamlet")
+            out.append(p)
+        return out
+    finally:
+        L.ora_destroy(ctx)

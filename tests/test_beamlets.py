@@ -82,6 +82,46 @@ def test_matrad_column_restatement_equals_reference_dose(oracle, tmp_path):
         oracle.free_problem(prob)
 
 
+def test_beamlet_primaries_geometry(oracle, tmp_path):
+    """The oracle's beamlet initHistory (omc_matrad.c:1084-1254; the MEX
+    cannot be built here) against geometry computed independently: the
+    primary starts on the phantom's bounding surface, moves along the unit
+    vector from the beam's source, on a line that crosses the bixel
+    parallelogram, in the voxel that contains its position."""
+    prob, src, bs = beamlet_problem(oracle, tmp_path, gantry=(0.0, 90.0, 135.0), field=4.0, bixel=2.0)
+    try:
+        xb, yb, zb, _ = phantom_arrays(prob)
+        lo = np.array([xb[0], yb[0], zb[0]])
+        hi = np.array([xb[-1], yb[-1], zb[-1]])
+        emax = 6.0
+        for b in range(src.nbeamlets):
+            s0 = bs.src[bs.ibeam[b]]
+            for p in oracle.beamlet_primaries(prob, src, b, range(40), seed=(3, b + 1)):
+                pos = np.array([p.x, p.y, p.z])
+                d = np.array([p.u, p.v, p.w])
+                assert p.iq == 0 and 0.0 < p.e <= emax and p.wt == 1.0
+                assert abs(np.linalg.norm(d) - 1.0) < 1e-12
+                # on the bounding box surface, inside its extent
+                assert np.all(pos >= lo - 1e-9) and np.all(pos <= hi + 1e-9)
+                assert np.min(np.minimum(np.abs(pos - lo), np.abs(pos - hi))) < 1e-9
+                # along the ray from the source
+                ray = pos - s0
+                assert np.linalg.norm(np.cross(ray / np.linalg.norm(ray), d)) < 1e-12 and ray @ d > 0
+                # the ray crosses the bixel: solve s0 + t d = corner + a side1 + c side2
+                M = np.column_stack([d, -bs.side1[b], -bs.side2[b]])
+                t, a, c = np.linalg.solve(M, bs.corner[b] - s0)
+                assert -1e-9 <= a <= 1 + 1e-9 and -1e-9 <= c <= 1 + 1e-9
+                # region index of the containing voxel (interior side of the face)
+                q = pos + 1e-9 * d
+                ix = int(np.clip(np.searchsorted(xb, q[0]) - 1, 0, len(xb) - 2))
+                iy = int(np.clip(np.searchsorted(yb, q[1]) - 1, 0, len(yb) - 2))
+                iz = int(np.clip(np.searchsorted(zb, q[2]) - 1, 0, len(zb) - 2))
+                ni, nj = len(xb) - 1, len(yb) - 1
+                assert p.ir == 1 + ix + iy * ni + iz * ni * nj
+    finally:
+        oracle.free_problem(prob)
+
+
 def test_beamlet_file_roundtrip_and_dij_reader(tmp_path):
     from ompmc_amd import formats as F
     bs = cfg.beamlets(gantry_deg=(0.0, 72.0), field=1.0, bixel=0.5)

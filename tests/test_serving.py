@@ -57,7 +57,7 @@ def test_request_result_same_alone_and_in_mix(D, problems):
         alone_c = drain(eng, [tc], {tc: nreg})[tc]
         # in a mix: B first (large: still being injected when the others
         # arrive), A and C arrive while B is on the device
-        tb = eng.submit(pid, pp.src, 2000000, nbatch=10, seed=(7, 7))
+        tb = eng.submit(pid, pp.src, 12000000, nbatch=10, seed=(7, 7))
         for _ in range(3):
             eng.step()
         ta2 = eng.submit(pid, pp.src, 20000, nbatch=10, seed=(1, 2))
