@@ -48,6 +48,12 @@ typedef struct {
                                 one launch before it continues in the next
                                 (its lane state travels with it, so results
                                 do not depend on this); 0 = automatic */
+    int     drain_lanes;     /* once a launch's input is used up, a wavefront
+                                with at most this many live lanes hands its
+                                particles to the next launch (exactly, as a
+                                yield); 0 = automatic, < 0 = never */
+    int     drain_trips;     /* ...after they have taken this many trips in
+                                the launch; 0 = automatic */
 } omc_dev_opts;
 
 /* Device counters of one run (all sums over the run). */

@@ -14,6 +14,7 @@ constexpr double RM = OMC_RM;
 constexpr int MXGE = OMC_MXGE;
 constexpr int MXEKE = OMC_MXEKE;
 constexpr int MAXMED = OMC_MXMED;
+constexpr int MAX_PHANTOMS = 16;     // resident phantoms per context
 
 // ---------------------------------------------------------------- RNG -----
 // Philox4x32-10 (Salmon et al., SC'11).  Each particle owns a stream: key
@@ -53,8 +54,12 @@ struct Rng {
             w0 = o[0]; w1 = o[1]; w2 = o[2]; w3 = o[3];
             blk = b;
         }
-        uint32_t s = ctr & 3u;
-        uint32_t w = s == 0 ? w0 : s == 1 ? w1 : s == 2 ? w2 : w3;
+        // word (ctr & 3) of the block by two selects on its bits (a select
+        // chain on the whole value becomes an indexed private array)
+        const uint32_t s = ctr & 3u;
+        const uint32_t even = (s & 2u) ? w2 : w0;
+        const uint32_t odd = (s & 2u) ? w3 : w1;
+        const uint32_t w = (s & 1u) ? odd : even;
         ctr++;
         return w >> 8;
     }

@@ -60,7 +60,6 @@ struct DevBuf {
     size_t bytes = 0;
 };
 
-constexpr int MAX_PHANTOMS = 16;
 constexpr int MAX_JOBS = 4096;
 constexpr int MAX_SEGS = 512;
 constexpr int MAX_PIPES = 8;
@@ -71,6 +70,12 @@ constexpr int DEFAULT_PIPES = 4;
 // the launch drains) and with overlapping pipelines.
 constexpr int64_t DEFAULT_CHUNK = 1 << 21;
 constexpr uint32_t DEFAULT_YIELD_TRIPS = 4096;
+// Drain compaction: once a launch's input is used up, a wavefront with at
+// most 32 live lanes hands its particles (after 16 trips in the launch) to
+// the next launch, which packs them into full wavefronts (slab sweep: FP32
+// 38 -> 55M histories/s, FP64 37 -> 42M).
+constexpr uint32_t DEFAULT_DRAIN_LANES = 32;
+constexpr uint32_t DEFAULT_DRAIN_TRIPS = 16;
 
 struct PhantomSlot {
     bool used = false;
@@ -168,6 +173,7 @@ struct omc_dev_ctx {
     int64_t chunk = 0;
     uint32_t qcap = 0;
     uint32_t yield_trips = 0;
+    uint32_t drain_lanes = 0, drain_trips = 0;
     int cus = 256;
     std::chrono::steady_clock::time_point t0;
 
@@ -707,6 +713,8 @@ template <typename R> static int launch_generation(omc_dev_ctx *c, int pi)
     A.qe = queue_view<R>(c, P, 2 + nxt);
     A.status = P.d_status;
     A.yield_trips = c->yield_trips;
+    A.drain_lanes = c->drain_lanes;
+    A.drain_trips = c->drain_trips;
     const int max_grid = c->cus * 8;
     P.had_ph = inP > 0;
     if (P.had_ph) {
@@ -927,7 +935,7 @@ int omc_dev_create(omc_dev_ctx **out, const omc_dev_opts *opts)
 {
     if (!out) return fail(OMC_EARG, "null context pointer");
     *out = nullptr;
-    omc_dev_opts o = { 0, OMC_PREC_FP64, 0, 0, 0, 0 };
+    omc_dev_opts o = { 0, OMC_PREC_FP64, 0, 0, 0, 0, 0, 0 };
     if (opts) o = *opts;
     if (o.precision != OMC_PREC_FP64 && o.precision != OMC_PREC_FP32)
         return fail(OMC_EARG, "unknown precision %d", o.precision);
@@ -943,6 +951,8 @@ int omc_dev_create(omc_dev_ctx **out, const omc_dev_opts *opts)
     c->device = o.device;
     c->precision = o.precision;
     c->yield_trips = o.yield_trips > 0 ? (uint32_t)o.yield_trips : DEFAULT_YIELD_TRIPS;
+    c->drain_lanes = o.drain_lanes > 0 ? (uint32_t)std::min(o.drain_lanes, 64) : o.drain_lanes < 0 ? 0u : DEFAULT_DRAIN_LANES;
+    c->drain_trips = o.drain_trips > 0 ? (uint32_t)o.drain_trips : DEFAULT_DRAIN_TRIPS;
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, o.device) == hipSuccess) c->cus = prop.multiProcessorCount;
     size_t freeb = 0, totalb = 0;

@@ -14,17 +14,17 @@ namespace omc {
 
 // ------------------------------------------------------------ math ------
 __device__ __forceinline__ double m_log(double x) { return log(x); }
-__device__ __forceinline__ float  m_log(float x)  { return logf(x); }
+__device__ __forceinline__ float  m_log(float x)  { return __logf(x); }
 __device__ __forceinline__ double m_exp(double x) { return exp(x); }
-__device__ __forceinline__ float  m_exp(float x)  { return expf(x); }
+__device__ __forceinline__ float  m_exp(float x)  { return __expf(x); }
 __device__ __forceinline__ double m_sqrt(double x) { return sqrt(x); }
-__device__ __forceinline__ float  m_sqrt(float x)  { return sqrtf(x); }
+__device__ __forceinline__ float  m_sqrt(float x)  { return __builtin_amdgcn_sqrtf(x); }
 __device__ __forceinline__ double m_sin(double x) { return sin(x); }
-__device__ __forceinline__ float  m_sin(float x)  { return sinf(x); }
+__device__ __forceinline__ float  m_sin(float x)  { return __sinf(x); }
 __device__ __forceinline__ double m_cos(double x) { return cos(x); }
-__device__ __forceinline__ float  m_cos(float x)  { return cosf(x); }
+__device__ __forceinline__ float  m_cos(float x)  { return __cosf(x); }
 __device__ __forceinline__ double m_pow(double x, double y) { return pow(x, y); }
-__device__ __forceinline__ float  m_pow(float x, float y)  { return powf(x, y); }
+__device__ __forceinline__ float  m_pow(float x, float y)  { return __powf(x, y); }
 __device__ __forceinline__ double m_fmax(double a, double b) { return fmax(a, b); }
 __device__ __forceinline__ float  m_fmax(float a, float b)  { return fmaxf(a, b); }
 __device__ __forceinline__ double m_fmin(double a, double b) { return fmin(a, b); }
@@ -39,15 +39,37 @@ template <typename R, typename T>
 __device__ __forceinline__ int pwlf_idx(R l, T c1, T c0) { return (int)(l * (R)c1 + (R)c0); }
 
 // --------------------------------------------------------- geometry -----
+// n / d for 0 <= n < 2^24, d > 0 without an integer divide: the float
+// quotient through the hardware reciprocal is off by at most a few units
+// for the smallest d (one for d > 4), and the integer correction loops
+// make it exact.
+__device__ __forceinline__ int small_div(int n, int d)
+{
+    int q = (int)((float)n * __builtin_amdgcn_rcpf((float)d));
+    int r = n - q * d;
+    while (r < 0) { q -= 1; r += d; }
+    while (r >= d) { q += 1; r -= d; }
+    return q;
+}
+
+// Voxel indices of region irl > 0 (region = 1 + ix + iy*imax + iz*ijmax),
+// the decomposition omc_dosxyz.c:205-207 computes with % and /.
+__device__ __forceinline__ void voxel_of(int irl, int imax, int ijmax, int &irx, int &iry, int &irz)
+{
+    int n = irl - 1;
+    irz = small_div(n, ijmax);
+    int rem = n - irz * ijmax;
+    iry = small_div(rem, imax);
+    irx = rem - iry * imax;
+}
 // howfar (omc_dosxyz.c:187-297) for a particle inside region irl > 0.
 template <typename R, typename Geo>
 __device__ __forceinline__ void howfar(const Geo &g, int irl, R x, R y, R z, R u, R v, R w,
                                        int &irnew, R &ustep)
 {
     int ijmax = g.isize * g.jsize, imax = g.isize;
-    int irx = (irl - 1) % imax;
-    int irz = (irl - 1 - irx) / ijmax;
-    int iry = ((irl - 1 - irx) - irz * ijmax) / imax;
+    int irx, iry, irz;
+    voxel_of(irl, imax, ijmax, irx, iry, irz);
     R dist;
     if (w > (R)0) {
         dist = ((R)g.zb[irz + 1] - z) / w;
@@ -78,9 +100,8 @@ __device__ __forceinline__ R hownear(const Geo &g, int irl, R x, R y, R z)
 {
     if (irl == 0) return (R)0;
     int ijmax = g.isize * g.jsize, imax = g.isize;
-    int irx = (irl - 1) % imax;
-    int irz = (irl - 1 - irx) / ijmax;
-    int iry = ((irl - 1 - irx) - irz * ijmax) / imax;
+    int irx, iry, irz;
+    voxel_of(irl, imax, ijmax, irx, iry, irz);
     R t = (R)1.0E10;
     t = m_fmin(t, (R)g.xb[irx + 1] - x);
     t = m_fmin(t, x - (R)g.xb[irx]);

@@ -118,7 +118,7 @@ __device__ __forceinline__ R spin_rejection(const DevTables<R> &T, Rng &rng, int
 
 // sscat (ompmc.c:3170-3199): one elastic scattering, spin rejection on
 template <typename R>
-__device__ void sscat(const DevTables<R> &T, Rng &rng, int imed, int qel, R chia2, R elke, R beta2,
+__device__ __forceinline__ void sscat(const DevTables<R> &T, Rng &rng, int imed, int qel, R chia2, R elke, R beta2,
                       R &cost, R &sint)
 {
     bool spin_index = true;
@@ -243,7 +243,7 @@ __device__ void mscat(const DevTables<R> &T, Rng &rng, int imed, int qel, bool &
 // msdist (ompmc.c:3787-3976): PRESTA-II substeps; moves p to the step end
 // position and direction, returns the straight-line distance.
 template <typename R>
-__device__ R msdist(const DevTables<R> &T, Particle<R> &p, int imed, R rhof, R de, R tustep, R eke)
+__device__ __forceinline__ R msdist(const DevTables<R> &T, Particle<R> &p, int imed, R rhof, R de, R tustep, R eke)
 {
     const MediumScalars &M = T.small->med[imed];
     const int qel = (1 + p.iq) / 2;
@@ -347,7 +347,7 @@ __device__ R msdist(const DevTables<R> &T, Particle<R> &p, int imed, R rhof, R d
 
 // rannih (ompmc.c:4111-4167): positron at rest -> a and b are the two photons
 template <typename R>
-__device__ void rannih(Particle<R> &a, Particle<R> &b)
+__device__ __forceinline__ void rannih(Particle<R> &a, Particle<R> &b)
 {
     R rnno = urand<R>(a.rng);
     R costhe = (R)2.0 * rnno - (R)1;

@@ -32,31 +32,41 @@ template <typename R> struct KArgs {
     Queue<R> qe;             // charged-particle output queue
     DevStatus *status;
     uint32_t yield_trips;    // loop trips a particle may take per launch
+    uint32_t drain_lanes;    // once the input is used up, a wave with at most
+                             // this many live lanes hands its particles on...
+    uint32_t drain_trips;    // ...if they have taken this many trips here
 };
 
-// Per-lane geometry / scoring context for the particle the lane holds.
+// Per-lane geometry / scoring context for the particle the lane holds.  The
+// phantom records are copied to LDS at kernel start, so a lane keeps one
+// pointer to its record instead of the record's fields.
 template <typename R> struct LaneGeo {
-    const DevPhantom<R> *g;
-    int isize, jsize, ksize;
-    const double *xb, *yb, *zb;
-    const uint8_t *med;
-    const R *rhof;
+    const DevPhantom<R> *g;  // LDS copy of the particle's phantom record
     double *tally;           // job tally row of this history's batch
     double *ledger;          // job ledger entry of this history (or null)
 
-    __device__ __forceinline__ void bind(const KArgs<R> &A, const Particle<R> &p)
+    __device__ __forceinline__ void bind(const KArgs<R> &A, const DevPhantom<R> *lds, const Particle<R> &p)
     {
         const DevJob &J = A.jobs[p.job];
-        g = &A.phantoms[J.phantom];
-        isize = g->isize; jsize = g->jsize; ksize = g->ksize;
-        xb = g->xb; yb = g->yb; zb = g->zb;
-        med = g->med; rhof = g->rhof;
+        g = &lds[J.phantom];
         uint32_t batch = (uint32_t)((uint64_t)p.hist / (uint64_t)J.nperbatch);
         tally = J.tally + (size_t)batch * (size_t)g->nreg;
         ledger = J.ledger ? J.ledger + 3 * (size_t)p.hist : nullptr;
     }
-    __device__ __forceinline__ int medium(int ir) const { int m = med[ir]; return m == 0xff ? -1 : m; }
+    __device__ __forceinline__ int medium(int ir) const { int m = g->med[ir]; return m == 0xff ? -1 : m; }
 };
+
+// Copies the resident phantom records to LDS (all threads of the block).
+template <typename R>
+__device__ __forceinline__ void stage_phantoms(const DevPhantom<R> *src, DevPhantom<R> *lds)
+{
+    constexpr int words = (int)(sizeof(DevPhantom<R>) * MAX_PHANTOMS / 4);
+    static_assert(sizeof(DevPhantom<R>) % 4 == 0, "phantom record must be word sized");
+    const uint32_t *s = reinterpret_cast<const uint32_t *>(src);
+    uint32_t *d = reinterpret_cast<uint32_t *>(lds);
+    for (int i = threadIdx.x; i < words; i += blockDim.x) d[i] = s[i];
+    __syncthreads();
+}
 
 // Deposit accumulator: consecutive deposits into the same region are summed
 // in a register and written with one FP64 atomic when the region changes.
@@ -148,7 +158,7 @@ template <typename R> struct PhotonLane {
 // Returns true while the photon is still owned by the lane.  Electrons it
 // sets in motion are pushed to A.qe.
 template <typename R>
-__device__ bool photon_iteration(const KArgs<R> &A, Particle<R> &p, PhotonLane<R> &s, LaneGeo<R> &G,
+__device__ __forceinline__ bool photon_iteration(const KArgs<R> &A, Particle<R> &p, PhotonLane<R> &s, LaneGeo<R> &G,
                                  Scorer<R> &sc, uint32_t &nsteps, uint32_t &ncross)
 {
     const DevTables<R> &T = A.T;
@@ -180,7 +190,7 @@ __device__ bool photon_iteration(const KArgs<R> &A, Particle<R> &p, PhotonLane<R
         s.lgle = pwlf_idx(s.gle, M.ge1, M.ge0) - 1;
         R4<R> c = T.ph_mfp[s.imed * MXGE + s.lgle];
         R gmfpr0 = s.gle * c.a1 + c.a0;
-        R rhof = G.rhof[irl];
+        R rhof = G.g->rhof[irl];
         s.gmfp = gmfpr0 / rhof;
         s.cohfac = s.gle * c.b1 + c.b0;
         s.gmfp *= s.cohfac;
@@ -194,7 +204,7 @@ __device__ bool photon_iteration(const KArgs<R> &A, Particle<R> &p, PhotonLane<R
         if (G.ledger) atomicAdd(&G.ledger[2], (double)p.wt * (double)p.e);
         return false;
     }
-    howfar<R>(G, irl, p.x, p.y, p.z, p.u, p.v, p.w, irnew, ustep);
+    howfar<R>(*G.g, irl, p.x, p.y, p.z, p.u, p.v, p.w, irnew, ustep);
     p.x += ustep * p.u;
     p.y += ustep * p.v;
     p.z += ustep * p.w;
@@ -336,8 +346,78 @@ __device__ __forceinline__ int electron_interaction(const KArgs<R> &A, Particle<
     return E_DONE;
 }
 
+// The ustep iteration that follows a step in matter while the tstep loop
+// goes on (ompmc.c:4780-5345).  total_tstep starts again at 0 in every
+// iteration (ompmc.c:4787), so here tstep = 0 and the step length is
+// min(0, tmxs): the msdist branch cannot be taken, one random number is
+// drawn for the single-scattering test, the particle does not move, and the
+// tstep loop restarts.  Run in the same trip as the step before it; when an
+// unusual table value would make the general iteration behave otherwise,
+// the general iteration runs instead in the next trip.
 template <typename R>
-__device__ int electron_iteration(const KArgs<R> &A, Particle<R> &p, ElectronLane<R> &s, LaneGeo<R> &G,
+__device__ __forceinline__ int zero_ustep(const KArgs<R> &A, Particle<R> &p, ElectronLane<R> &s, LaneGeo<R> &G,
+                                          Scorer<R> &sc, uint32_t &nsteps, uint32_t &ncross)
+{
+    if (s.compute_tstep || !(s.sig0 > (R)0.0) || s.imed == -1) return E_ALIVE;
+    const DevTables<R> &T = A.T;
+    const R rm = (R)RM;
+    const int iq = p.iq;
+    const int qel = (1 + iq) / 2;
+    const R eie = p.e;
+    int irl = p.ir;
+    const int imed = s.imed;
+    const MediumScalars *M = &T.small->med[imed];
+    const R rhof = G.g->rhof[irl];
+    const R tstep = (R)(0.0 / (double)rhof);
+    const ElecBin<R> &eb = T.el[(imed * 2 + qel) * MXEKE + s.lelke];
+    const ElecCsda &cs = T.csda[(imed * 2 + qel) * MXEKE + s.lelke];
+    R dedx0 = s.elke * (R)cs.dedx1 + (R)cs.dedx0;
+    R dedx = rhof * dedx0;
+    R tmxs = s.elke * eb.tmxs1 + eb.tmxs0;
+    tmxs /= rhof;
+    // the range term of min(tstep, tmxs, range) is >= 0 and cannot lower it
+    R tustep = m_fmin(tstep, tmxs);
+    R tperp = hownear<R>(*G.g, irl, p.x, p.y, p.z);
+    R blccl = rhof * (R)M->blcc;
+    R xccl = rhof * (R)M->xcc;
+    R p2 = s.eke * (s.eke + (R)2.0 * rm);
+    R beta2 = p2 / (p2 + sq(rm));
+    R etap = s.elke * eb.eta1 + eb.eta0;
+    R ms_corr = s.elke * eb.blc1 + eb.blc0;
+    blccl = blccl / etap / ((R)1.0 + (R)0.25 * etap * xccl / blccl / p2) * ms_corr;
+    R ssmfp = beta2 / blccl;
+    R skindepth = (R)3 * ssmfp;
+    if (!(skindepth > (R)0.0)) return E_ALIVE;
+    tustep = m_fmin(tustep, m_fmax(tperp, skindepth));
+    // from here on the general iteration's path is fixed
+    nsteps++;
+    s.rhof = rhof;
+    R rnno = urand<R>(p.rng);
+    if (rnno < (R)1.0E-30) rnno = (R)1.0E-30;
+    R lambda = (R)(-1.0) * m_log((R)1.0 - rnno);
+    R lambda_max = (R)0.5 * blccl * rm / dedx;
+    lambda_max *= (s.eke / rm + (R)1.0) * (s.eke / rm + (R)1.0) * (s.eke / rm + (R)1.0);
+    if (!(lambda >= (R)0.0 && lambda_max > (R)0.0)) return E_DONE;   // dropped unscored
+    // tuss >= 0 >= tustep: single scattering stays off
+    R ustep = tustep;
+    int irnew = irl;
+    const int irold = irl;
+    if (!(ustep < tperp)) howfar<R>(*G.g, irl, p.x, p.y, p.z, p.u, p.v, p.w, irnew, ustep);
+    if (ustep < (R)0) ustep = (R)0.0;
+    s.ustep = ustep;
+    if (irnew != irold) {
+        ncross++;
+        p.ir = irnew;
+        irl = irnew;
+        s.imed = G.medium(irl);
+    }
+    if (eie <= (R)G.g->ecut[s.imed + 1]) { electron_cutoff<R>(A, p, G, sc, eie); return E_DONE; }
+    s.phase = 1;
+    return E_ALIVE;
+}
+
+template <typename R>
+__device__ __forceinline__ int electron_iteration(const KArgs<R> &A, Particle<R> &p, ElectronLane<R> &s, LaneGeo<R> &G,
                                   Scorer<R> &sc, uint32_t &nsteps, uint32_t &ncross)
 {
     const DevTables<R> &T = A.T;
@@ -349,7 +429,7 @@ __device__ int electron_iteration(const KArgs<R> &A, Particle<R> &p, ElectronLan
     if (s.phase == 0) {
         int irl = p.ir;
         s.imed = G.medium(irl);
-        s.rhof = G.rhof[irl];
+        s.rhof = G.g->rhof[irl];
         s.medold = s.imed;
         if (eie <= (R)G.g->ecut[s.imed + 1]) { electron_cutoff<R>(A, p, G, sc, eie); return E_DONE; }
         s.phase = 1;
@@ -395,7 +475,7 @@ __device__ int electron_iteration(const KArgs<R> &A, Particle<R> &p, ElectronLan
     } else {
         csb = T.csda + (imed * 2 + qel) * MXEKE;
         const double eke0 = M->eke0, eke1 = M->eke1;
-        s.rhof = G.rhof[irl];
+        s.rhof = G.g->rhof[irl];
         const R rhof = s.rhof;
         if (s.sig0 <= (R)0.0) {
             tstep = (R)10.0E8;
@@ -443,7 +523,7 @@ __device__ int electron_iteration(const KArgs<R> &A, Particle<R> &p, ElectronLan
             range = (R)rg;
         }
         tustep = m_fmin(m_fmin(tstep, tmxs), range);
-        tperp = hownear<R>(G, irl, p.x, p.y, p.z);
+        tperp = hownear<R>(*G.g, irl, p.x, p.y, p.z);
         R blccl = rhof * (R)M->blcc;
         R xccl = rhof * (R)M->xcc;
         R p2 = s.eke * (s.eke + (R)2.0 * rm);
@@ -501,7 +581,7 @@ __device__ int electron_iteration(const KArgs<R> &A, Particle<R> &p, ElectronLan
             }
             return E_DONE;
         }
-        howfar<R>(G, irl, p.x, p.y, p.z, p.u, p.v, p.w, irnew, ustep);
+        howfar<R>(*G.g, irl, p.x, p.y, p.z, p.u, p.v, p.w, irnew, ustep);
     }
     if (ustep < (R)0) ustep = (R)0.0;
     s.ustep = ustep;
@@ -594,15 +674,39 @@ __device__ int electron_iteration(const KArgs<R> &A, Particle<R> &p, ElectronLan
     s.demfp -= de * s.sig0;
     total_tstep -= (double)tvstep * (double)rhof;
     if (total_tstep < 1.0E-9) s.demfp = (R)0.0;
-    if (s.demfp >= (R)EPSEMFP) return E_ALIVE;      // next ustep iteration
+    if (s.demfp >= (R)EPSEMFP) return zero_ustep<R>(A, p, s, G, sc, nsteps, ncross);
 
     return electron_interaction<R>(A, p, s);
 }
 
 // ============================================================ kernels =====
+// Minimum waves per SIMD the register allocation of each transport kernel
+// must allow (the second __launch_bounds__ argument); values above what the
+// kernel needs make the compiler keep fewer values in VGPRs.  Chosen by a
+// sweep on the slab phantom (16M histories): FP32 photon 4 / electron 3
+// (no spills), FP64 photon 2 / electron 2 (a few spills, still faster than
+// one wave).
+#ifndef OMC_WAVES_P32
+#define OMC_WAVES_P32 4
+#endif
+#ifndef OMC_WAVES_E32
+#define OMC_WAVES_E32 3
+#endif
+#ifndef OMC_WAVES_P64
+#define OMC_WAVES_P64 2
+#endif
+#ifndef OMC_WAVES_E64
+#define OMC_WAVES_E64 2
+#endif
+template <typename R> struct Waves;
+template <> struct Waves<float> { static constexpr int photon = OMC_WAVES_P32, electron = OMC_WAVES_E32; };
+template <> struct Waves<double> { static constexpr int photon = OMC_WAVES_P64, electron = OMC_WAVES_E64; };
+
 template <typename R>
-__global__ void __launch_bounds__(256) k_photon(KArgs<R> A)
+__global__ void __launch_bounds__(256, Waves<R>::photon) k_photon(KArgs<R> A)
 {
+    __shared__ DevPhantom<R> lds_ph[MAX_PHANTOMS];
+    stage_phantoms<R>(A.phantoms, lds_ph);
     Particle<R> p;
     PhotonLane<R> s;
     LaneGeo<R> G;
@@ -621,21 +725,26 @@ __global__ void __launch_bounds__(256) k_photon(KArgs<R> A)
                     s.phase = 0;
                     trips = 0;
                 }
-                G.bind(A, p);
+                G.bind(A, lds_ph, p);
                 here = 0;
                 alive = true;
             } else {
                 exhausted = true;
             }
         }
-        if (__ballot(alive) == 0) break;
+        const unsigned long long live = __ballot(alive);
+        if (live == 0) break;
+        // input used up and the wave mostly empty: hand the rest on, the
+        // next launch packs them into full waves
+        const bool drain = __ballot(exhausted) != 0 && (uint32_t)__popcll(live) <= A.drain_lanes;
         if (alive) {
             alive = photon_iteration<R>(A, p, s, G, sc, nsteps, ncross);
             ++trips;
+            ++here;
             if (alive && trips > MAX_ITERS) {           // watchdog: no particle runs forever
                 atomicAdd(&A.status->killed, 1u);
                 alive = false;
-            } else if (alive && ++here >= A.yield_trips) {
+            } else if (alive && (here >= A.yield_trips || (drain && here >= A.drain_trips))) {
                 sc.flush(G);
                 yield_push<R>(A.qp, p, s, trips, A.status);
                 nyield++;
@@ -652,8 +761,10 @@ __global__ void __launch_bounds__(256) k_photon(KArgs<R> A)
 }
 
 template <typename R>
-__global__ void __launch_bounds__(256) k_electron(KArgs<R> A)
+__global__ void __launch_bounds__(256, Waves<R>::electron) k_electron(KArgs<R> A)
 {
+    __shared__ DevPhantom<R> lds_ph[MAX_PHANTOMS];
+    stage_phantoms<R>(A.phantoms, lds_ph);
     Particle<R> p;
     ElectronLane<R> s;
     LaneGeo<R> G;
@@ -672,21 +783,24 @@ __global__ void __launch_bounds__(256) k_electron(KArgs<R> A)
                     s.phase = 0;
                     trips = 0;
                 }
-                G.bind(A, p);
+                G.bind(A, lds_ph, p);
                 here = 0;
                 alive = true;
             } else {
                 exhausted = true;
             }
         }
-        if (__ballot(alive) == 0) break;
+        const unsigned long long live = __ballot(alive);
+        if (live == 0) break;
+        const bool drain = __ballot(exhausted) != 0 && (uint32_t)__popcll(live) <= A.drain_lanes;
         if (alive) {
             int r = electron_iteration<R>(A, p, s, G, sc, nsteps, ncross);
             ++trips;
+            ++here;
             if (r != E_DONE && trips > MAX_ITERS) {     // watchdog: no particle runs forever
                 atomicAdd(&A.status->killed, 1u);
                 r = E_DONE;
-            } else if (r != E_DONE && ++here >= A.yield_trips) {
+            } else if (r != E_DONE && (here >= A.yield_trips || (drain && here >= A.drain_trips))) {
                 sc.flush(G);
                 yield_push<R>(A.qe, p, s, trips, A.status);
                 nyield++;

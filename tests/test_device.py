@@ -153,14 +153,17 @@ def test_schedule_and_reruns_do_not_change_histories(oracle, D, tmp_path, case, 
     """Histories are identified by (seed, index) only: a different number of
     pipelines, a small refill chunk (many injections, primaries mixed with
     older secondaries), particles handed to the next launch after every
-    one or three loop trips (their lane state travels with them), or a
-    repeat reproduce the same energies up to FP64 summation order."""
+    one or three loop trips (their lane state travels with them), handed
+    on after every trip once a launch's input is used up, or a repeat
+    reproduce the same energies up to FP64 summation order."""
     stem = write_case(case, tmp_path, ncase=3000)
     prob = oracle.load_problem(stem)
     try:
         out = []
-        for chunk, pipes, ytrips in ((0, 0, 0), (0, 1, 0), (777, 3, 0), (0, 2, 1), (5000, 0, 3)):
-            with D.Engine(precision, chunk=chunk, pipelines=pipes, yield_trips=ytrips) as eng:
+        for chunk, pipes, ytrips, drain in ((0, 0, 0, -1), (0, 1, 0, 0), (777, 3, 0, 0), (0, 2, 1, 0),
+                                            (5000, 0, 3, 0), (2000, 2, 0, 64)):
+            with D.Engine(precision, chunk=chunk, pipelines=pipes, yield_trips=ytrips, drain_lanes=drain,
+                          drain_trips=1 if drain == 64 else 0) as eng:
                 eng.upload_media(prob.tab)
                 pid = eng.upload_phantom(prob.ph, prob.reg)
                 s, _, st, led = eng.run_dosxyz(pid, prob.src, prob.nperbatch, prob.nbatch, seed=(4, 4),
@@ -169,7 +172,7 @@ def test_schedule_and_reruns_do_not_change_histories(oracle, D, tmp_path, case, 
     finally:
         oracle.free_problem(prob)
     assert out[2][2] >= 4                       # 3000 histories, chunk 777
-    assert out[3][3] > 1000 and out[4][3] > 100
+    assert out[3][3] > 1000 and out[4][3] > 100 and out[5][3] > 100
     for s, led, _, _ in out[1:]:
         assert np.allclose(s, out[0][0], rtol=1e-12, atol=1e-12 * out[0][0].max())
         assert np.allclose(led, out[0][1], rtol=1e-12, atol=1e-12)
