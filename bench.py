@@ -44,6 +44,12 @@ import numpy as np
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
+# The engine drives 4 pipeline streams plus a helper stream; with HIP's
+# default of 4 hardware queues per process two of them can share a queue and
+# serialise.  Give the process 8 before the HIP runtime starts.
+if int(os.environ.get("GPU_MAX_HW_QUEUES", "0") or 0) < 8:
+    os.environ["GPU_MAX_HW_QUEUES"] = "8"
+
 HBM_PEAK_GBS = 8000.0
 METRIC = "histories/s and p50/p99 beamlet-dose latency at 2% σ, FP32-batched vs FP64 GPU"
 
@@ -89,8 +95,12 @@ def time_slab(prob, precision, hist, steps, warmup, pipelines=0, chunk=0):
     with D.Engine(precision, pipelines=pipelines, chunk=chunk) as eng:
         eng.upload_media(prob.tab)
         pid = eng.upload_phantom(prob.ph, prob.reg)
-        for w in range(warmup):
+        # untimed warm-up: the requested steps, and at least 3 s of transport
+        # so that clocks and caches have settled before the timed region
+        t_w, w = time.perf_counter(), 0
+        while w < warmup or time.perf_counter() - t_w < 3.0:
             eng.run_dosxyz(pid, prob.src, nper, nbatch, seed=(1000 + w, 7), nreg=nreg)
+            w += 1
         stats, etot = [], []
         t0 = time.perf_counter()
         for k in range(steps):
@@ -99,6 +109,19 @@ def time_slab(prob, precision, hist, steps, warmup, pipelines=0, chunk=0):
             etot.append(float(s[1:].sum()))
         el = time.perf_counter() - t0
     return el, stats, etot, nper * nbatch
+
+
+def profiled_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary (copy-calibrated), with the algorithmic bytes per
+    launch of that same profiled run."""
+    f = ROOT / "profiles" / f"{kernel}_slab6mv_fp32_traffic.md"
+    if not f.exists():
+        return None, None
+    text = f.read_text()
+    t = re.search(r"traffic_bytes_per_launch: ([0-9.e+]+)", text)
+    a = re.search(r"algorithmic_bytes_per_launch: ([0-9.e+]+)", text)
+    return (float(t.group(1)), float(a.group(1))) if t and a else (None, None)
 
 
 def roofline(stats, precision):
@@ -116,9 +139,13 @@ def roofline(stats, precision):
     k = kern[name]
     achieved = k["bytes"] / (k["ms"] / 1e3) / 1e9 if k["ms"] > 0 else 0.0
     hist = tot["histories"]
+    traffic, alg_profiled = profiled_traffic(name) if precision == 1 else (None, None)
     return {
         "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-        "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+        "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+        "traffic_over_algorithmic": (traffic / alg_profiled) if traffic else None,
+        "traffic_source": (f"profiles/{name}_slab6mv_fp32_traffic.md (1e6-history profiled run; bytes per launch)"
+                           if traffic else None),
         "kernel": name, "launches": k["launches"],
         "algorithmic_bytes_per_launch": k["bytes"] / max(k["launches"], 1),
         "avg_launch_ms": k["ms"] / max(k["launches"], 1),

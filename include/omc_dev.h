@@ -74,6 +74,10 @@ typedef struct {
     int64_t photon_crossings, electron_crossings;   /* voxel boundaries crossed */
     int64_t photon_deposits, electron_deposits;     /* ausgab calls per kernel */
     int64_t yields;             /* particles handed on to a later launch */
+    double  host_ms;            /* scheduler thread busy (admission, launches,
+                                   generation ends), not waiting */
+    double  pipe_idle_ms;       /* summed over pipelines: device time between a
+                                   generation's end and the next one's start */
 } omc_dev_stats;
 
 /* Context: device, pipelines (each a stream with its own particle queues),

@@ -102,7 +102,9 @@ struct Pipe {
     SourceSeg *d_segs = nullptr;  // [MAX_SEGS]
     PipeMirror *h = nullptr;
     hipEvent_t ev_src0 = nullptr, ev_src1 = nullptr, ev_ph0 = nullptr, ev_ph1 = nullptr;
-    hipEvent_t ev_el0 = nullptr, ev_el1 = nullptr, ev_done = nullptr;
+    hipEvent_t ev_el0 = nullptr, ev_el1 = nullptr, ev_begin = nullptr;
+    hipEvent_t ev_end[2] = { nullptr, nullptr };   // end of generation gen (slot gen & 1)
+    uint64_t gen = 0;             // generations launched on this pipeline
     int cur = 0;
     uint32_t nP = 0, nE = 0;      // particles waiting in the current queues
     bool inflight = false;
@@ -280,7 +282,8 @@ template <typename R> Queue<R> queue_view(const omc_dev_ctx *c, const Pipe &P, i
 template <typename R> int alloc_pipe(omc_dev_ctx *c, Pipe &P)
 {
     if (hipStreamCreateWithFlags(&P.stream, hipStreamNonBlocking) != hipSuccess) return fail(OMC_EDEVICE, "stream");
-    hipEvent_t *evs[] = { &P.ev_src0, &P.ev_src1, &P.ev_ph0, &P.ev_ph1, &P.ev_el0, &P.ev_el1, &P.ev_done };
+    hipEvent_t *evs[] = { &P.ev_src0, &P.ev_src1, &P.ev_ph0, &P.ev_ph1, &P.ev_el0, &P.ev_el1, &P.ev_begin,
+                          &P.ev_end[0], &P.ev_end[1] };
     for (auto *e : evs) HIPCHK(hipEventCreate(e));
     for (int k = 0; k < 4; ++k) {
         for (int f = 0; f < 8; ++f) OMC_TRY_DEV(dalloc(c, &P.q[k][f], sizeof(R) * (size_t)c->qcap));
@@ -308,7 +311,8 @@ void free_pipe(Pipe &P)
     void *ptrs[] = { P.d_counts, P.d_status, P.d_flags, P.d_segs };
     for (void *p : ptrs) if (p) (void)hipFree(p);
     if (P.h) (void)hipHostFree(P.h);
-    hipEvent_t evs[] = { P.ev_src0, P.ev_src1, P.ev_ph0, P.ev_ph1, P.ev_el0, P.ev_el1, P.ev_done };
+    hipEvent_t evs[] = { P.ev_src0, P.ev_src1, P.ev_ph0, P.ev_ph1, P.ev_el0, P.ev_el1, P.ev_begin,
+                         P.ev_end[0], P.ev_end[1] };
     for (auto e : evs) if (e) (void)hipEventDestroy(e);
     if (P.stream) (void)hipStreamDestroy(P.stream);
 }
@@ -695,6 +699,7 @@ template <typename R> static int launch_generation(omc_dev_ctx *c, int pi)
     const uint32_t ntot = injP + injE;
     const uint32_t inP = P.nP + injP, inE = P.nE + injE;
     if (!inP && !inE) return OMC_OK;
+    HIPCHK(hipEventRecord(P.ev_begin, s));
     launch_gen_reset(P.d_counts, nxt, s);
     const Queue<R> qpc = queue_view<R>(c, P, P.cur), qec = queue_view<R>(c, P, 2 + P.cur);
     P.had_src = nseg > 0;
@@ -742,7 +747,7 @@ template <typename R> static int launch_generation(omc_dev_ctx *c, int pi)
     HIPCHK(hipMemcpyAsync(P.h->counts, P.d_counts, 8 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     HIPCHK(hipMemcpyAsync(&P.h->status, P.d_status, sizeof(DevStatus), hipMemcpyDeviceToHost, s));
     HIPCHK(hipMemcpyAsync(P.h->flags, P.d_flags, njobs * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-    HIPCHK(hipEventRecord(P.ev_done, s));
+    HIPCHK(hipEventRecord(P.ev_end[P.gen & 1], s));
     P.injected = ntot;
     P.inflight = true;
     return OMC_OK;
@@ -757,6 +762,12 @@ static int complete_generation(omc_dev_ctx *c, int pi)
     P.nE = std::min(P.h->counts[2 + P.cur], c->qcap);
     omc_dev_stats &S = c->acc;
     float ms = 0.f;
+    if (P.gen > 0) {
+        // stream idle between the previous generation's end and this one's start
+        HIPCHK(hipEventElapsedTime(&ms, P.ev_end[(P.gen - 1) & 1], P.ev_begin));
+        S.pipe_idle_ms += ms;
+    }
+    P.gen++;
     if (P.had_src) {
         HIPCHK(hipEventElapsedTime(&ms, P.ev_src0, P.ev_src1));
         S.source_kernel_ms += ms;
@@ -835,9 +846,11 @@ static void reset_engine(omc_dev_ctx *c, int rc)
 // generation was processed, 0 if the engine is idle.
 template <typename R> static int turn(omc_dev_ctx *c)
 {
+    auto h0 = std::chrono::steady_clock::now();
     int rc = admit(c);
     for (size_t p = 0; !rc && p < c->pipes.size(); ++p)
         if (!c->pipes[p].inflight) rc = launch_generation<R>(c, (int)p);
+    c->acc.host_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - h0).count();
     if (rc) { reset_engine(c, rc); return rc; }
     bool any = false;
     for (auto &P : c->pipes) any |= P.inflight;
@@ -848,7 +861,7 @@ template <typename R> static int turn(omc_dev_ctx *c)
         for (size_t p = 0; p < c->pipes.size(); ++p) {
             Pipe &P = c->pipes[p];
             if (!P.inflight) continue;
-            hipError_t e = hipEventQuery(P.ev_done);
+            hipError_t e = hipEventQuery(P.ev_end[P.gen & 1]);
             if (e == hipSuccess) { oldest = (int)p; break; }
             if (e != hipErrorNotReady) {
                 rc = fail(OMC_EDEVICE, "pipeline %zu: %s", p, hipGetErrorString(e));
@@ -857,12 +870,17 @@ template <typename R> static int turn(omc_dev_ctx *c)
             }
         }
         if (oldest >= 0) {
+            auto h1 = std::chrono::steady_clock::now();
             rc = complete_generation(c, oldest);
+            c->acc.host_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - h1).count();
             if (rc) { reset_engine(c, rc); return rc; }
             return 1;
         }
-        if (++spins < 2000) std::this_thread::yield();
-        else std::this_thread::sleep_for(std::chrono::microseconds(50));
+        // generations end every few milliseconds: keep the host thread
+        // polling so the next one starts at once, and back off only when
+        // nothing has ended for a long time
+        if (++spins < 200000) std::this_thread::yield();
+        else std::this_thread::sleep_for(std::chrono::microseconds(20));
     }
 }
 

@@ -27,7 +27,8 @@ class DevStats(C.Structure):
                 ("source_kernel_ms", C.c_double), ("photon_launches", C.c_int64),
                 ("electron_launches", C.c_int64), ("killed", C.c_int64), ("photon_crossings", C.c_int64),
                 ("electron_crossings", C.c_int64), ("photon_deposits", C.c_int64),
-                ("electron_deposits", C.c_int64), ("yields", C.c_int64)]
+                ("electron_deposits", C.c_int64), ("yields", C.c_int64), ("host_ms", C.c_double),
+                ("pipe_idle_ms", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
