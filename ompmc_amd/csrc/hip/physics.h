@@ -38,6 +38,20 @@ template <typename R> __device__ __forceinline__ R sq(R x) { return x * x; }
 template <typename R, typename T>
 __device__ __forceinline__ int pwlf_idx(R l, T c1, T c0) { return (int)(l * (R)c1 + (R)c0); }
 
+// Table bin, the reference's pwlfInterval - 1 (ompmc.c:1954, 4729), clamped to
+// [0, nbins - 1].  l*c1 and c0 nearly cancel at a table's lower edge (c0 is
+// about 1070 for the photon tables), so in FP32 an energy within ~5e-7
+// (relative) above the edge -- a photon just above pcut, an electron just
+// above ecut - rm -- rounds to bin -1 and would read the preceding medium's
+// row, or memory before the table for medium 0.  The exact index there is
+// bin 0, which the clamp returns.
+template <typename R, typename T>
+__device__ __forceinline__ int pwlf_bin(R l, T c1, T c0, int nbins)
+{
+    int i = (int)(l * (R)c1 + (R)c0) - 1;
+    return min(max(i, 0), nbins - 1);
+}
+
 // --------------------------------------------------------- geometry -----
 // n / d for 0 <= n < 2^24, d > 0 without an integer divide: the float
 // quotient through the hardware reciprocal is off by at most a few units

@@ -187,7 +187,7 @@ __device__ __forceinline__ bool photon_iteration(const KArgs<R> &A, Particle<R> 
     R tstep;
     if (s.imed != -1) {
         const MediumScalars &M = T.small->med[s.imed];
-        s.lgle = pwlf_idx(s.gle, M.ge1, M.ge0) - 1;
+        s.lgle = pwlf_bin(s.gle, M.ge1, M.ge0, MXGE);
         R4<R> c = T.ph_mfp[s.imed * MXGE + s.lgle];
         R gmfpr0 = s.gle * c.a1 + c.a0;
         R rhof = G.g->rhof[irl];
@@ -443,7 +443,7 @@ __device__ __forceinline__ int electron_iteration(const KArgs<R> &A, Particle<R>
             if (rnno == (R)0.0) rnno = (R)1.0E-30;
             s.demfp = m_fmax(-m_log(rnno), (R)EPSEMFP);
             s.elke = m_log(s.eke);
-            s.lelke = pwlf_idx(s.elke, M->eke1, M->eke0) - 1;
+            s.lelke = pwlf_bin(s.elke, M->eke1, M->eke0, MXEKE);
             if (qel == 0 ? M->mono_e : M->mono_p) {
                 const ElecBin<R> &eb = T.el[(s.imed * 2 + qel) * MXEKE + s.lelke];
                 const ElecCsda &cs = T.csda[(s.imed * 2 + qel) * MXEKE + s.lelke];
@@ -490,7 +490,7 @@ __device__ __forceinline__ int electron_iteration(const KArgs<R> &A, Particle<R>
                     ts = 10.0E8;
                 } else {
                     double elkef = log(ekef);
-                    int lelkef = (int)(elkef * eke1 + eke0) - 1;
+                    int lelkef = pwlf_bin(elkef, eke1, eke0, MXEKE);
                     if (lelkef == s.lelke) {
                         ts = drange(&csb[s.lelke], iq, eke, ekef, elke, elkef);
                     } else {
@@ -632,7 +632,7 @@ __device__ __forceinline__ int electron_iteration(const KArgs<R> &A, Particle<R>
             R beta2 = p2 / (p2 + sq(rm));
             R chia2 = (R)M->xcc / ((R)4.0 * (R)M->blcc * p2);
             R elkems = m_log(ekems);
-            int lelkems = pwlf_idx(elkems, M->eke1, M->eke0) - 1;
+            int lelkems = pwlf_bin(elkems, M->eke1, M->eke0, MXEKE);
             const ElecBin<R> &ebs = T.el[(imed * 2 + qel) * MXEKE + lelkems];
             R etap = elkems * ebs.eta1 + ebs.eta0;
             chia2 *= etap;
@@ -661,7 +661,7 @@ __device__ __forceinline__ int electron_iteration(const KArgs<R> &A, Particle<R>
     if (imed != -1) {
         s.eke = eie - rm;
         s.elke = m_log(s.eke);
-        s.lelke = pwlf_idx(s.elke, M->eke1, M->eke0) - 1;
+        s.lelke = pwlf_bin(s.elke, M->eke1, M->eke0, MXEKE);
     }
     if (irnew != irold) {
         ncross++;

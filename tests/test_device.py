@@ -198,6 +198,43 @@ def test_vacuum_voxels_inside_phantom(oracle, D, tmp_path, energy, charge):
     assert same.mean() >= 0.995, f"{np.count_nonzero(~same)} histories differ"
 
 
+@pytest.mark.parametrize("charge,edge", [(0, 0.01), (-1, 0.7 - RM)])
+@pytest.mark.parametrize("ulps", [1, 2, 3, 5, 8])
+def test_fp32_energies_on_table_lower_edge(oracle, D, tmp_path, charge, edge, ulps):
+    """Particles a few FP32 ulps above a table's lower edge: a photon just
+    above pcut = ap = 10 keV, an electron just above ecut - m.  There the
+    table index l*c1 + c0 nearly cancels (c0 ~ 1070 for the photon tables),
+    and FP32 rounding put it one bin below the table, reading another
+    medium's coefficients; at 10 keV that made a photon Rayleigh-scatter
+    until the trip watchdog killed it.  Every history must now end by itself
+    and conserve its energy."""
+    from ompmc_amd import configs as cfg
+    e = np.float32(edge)
+    for _ in range(ulps):
+        e = np.nextafter(e, np.float32(1))
+    ph = cfg.water_cube(10, 5.0)
+    pf = ph.write_egsphant(tmp_path / "edge.egsphant")
+    items = cfg.dosxyz_inp_items(tmp_path, pf, ncase=2000, mono=f"{float(e):.9g}", charge=charge,
+                                 field=(-1, 1, -1, 1))
+    cfg.write_inp(tmp_path / "edge", items)
+    prob = oracle.load_problem(tmp_path / "edge")
+    try:
+        with D.Engine(1) as eng:
+            eng.upload_media(prob.tab)
+            pid = eng.upload_phantom(prob.ph, prob.reg)
+            s, _, st, led = eng.run_dosxyz(pid, prob.src, prob.nperbatch, prob.nbatch, nreg=prob.reg.nreg,
+                                           ledger=True)
+    finally:
+        oracle.free_problem(prob)
+    assert st.killed == 0
+    tot = led.reshape(-1, 3).sum(1)
+    # FP32 carries an electron's total energy T + m (m = 0.511 MeV), so each
+    # electron set in motion rounds by up to half an ulp of m (3e-8 MeV):
+    # an absolute bound, 3e-6 of a 10 keV history.  A history cut short
+    # loses most of its energy.
+    assert np.allclose(tot, float(e), rtol=0, atol=8 * 2.0**-24)
+
+
 def test_error_paths_return_codes(oracle, D, tmp_path):
     stem = write_case("water2MeV", tmp_path, ncase=100)
     prob = oracle.load_problem(stem)
