@@ -17,7 +17,8 @@ Also reported:
   beamlets     matRad-style dose influence on the 128x128x96 patient phantom:
                400 beamlets of 0.5x0.5 cm^2 at gantry 0, histories
                log-uniform in [1e4, 1e6], all in one device pass with the
-               sparse Dij (1% threshold) assembled on the device; FP32 and FP64
+               sparse Dij (1% threshold) assembled on the device; FP32 and FP64,
+               and the plain FP64 method (one beamlet per job, in turn)
   latency      Poisson stream of single-beamlet requests, each sized for 2%
                relative sigma at its Dmax voxel, served by continuous batching;
                p50/p99 of submit -> result, FP32 and FP64 at the same load
@@ -174,7 +175,38 @@ def beamlet_batch(prob, src, nhist, precision, nbatch=10):
             "device_transport_ms": st.transport_ms}, (jcs, irs, vals)
 
 
+def beamlet_plain(prob, src, nhist, precision, nbatch=10):
+    """The plain method: one beamlet per device job, each run to completion
+    before the next is submitted (no batching across beamlets).  Only the
+    job status is polled (no dose copied back), which favours this path."""
+    from ompmc_amd import device as D
+    with D.Engine(precision) as eng:
+        eng.upload_media(prob.tab)
+        pid = eng.upload_phantom(prob.ph, prob.reg)
+        wait_done(eng, eng.submit(pid, src, 20000, nbatch=nbatch, seed=(5, 5), kind=1, beamlet=0))   # warm-up
+        t0 = time.perf_counter()
+        total = 0
+        for b in range(src.nbeamlets):
+            res = wait_done(eng, eng.submit(pid, src, int(nhist[b]), nbatch=nbatch, seed=(6, 6 + b), kind=1,
+                                            beamlet=b))
+            if res.status != 0:
+                raise RuntimeError(f"beamlet {b} failed: {res.status}")
+            total += res.nhist
+        el = time.perf_counter() - t0
+    return {"seconds": el, "histories": total, "histories_per_s": total / el,
+            "method": "one beamlet per job, run to completion in turn"}
+
+
 # --------------------------------------------------------------- latency --
+def wait_done(eng, ticket):
+    """Steps the engine until the ticket is done; returns its result record."""
+    while True:
+        res, _, _ = eng.poll(ticket)
+        if res.done:
+            return res
+        eng.step()
+
+
 def wait_ticket(eng, ticket, nreg):
     """Steps the engine until the ticket is done; returns (result, sum, sum2)."""
     while True:
@@ -371,7 +403,10 @@ def main():
             for name, prec in (("fp32", D.FP32), ("fp64", D.FP64)):
                 log(f"beamlets {name}: {src.nbeamlets} beamlets, {int(bl_nh.sum())} histories")
                 res[name], _ = beamlet_batch(pprob, src, bl_nh, prec)
+            log(f"beamlets plain fp64: {src.nbeamlets} jobs in turn")
+            res["plain_fp64"] = beamlet_plain(pprob, src, bl_nh, D.FP64)
             res["fp32_over_fp64"] = res["fp32"]["histories_per_s"] / res["fp64"]["histories_per_s"]
+            res["fp32_batched_over_plain_fp64"] = res["fp32"]["histories_per_s"] / res["plain_fp64"]["histories_per_s"]
             res["workload"] = ("patient 128x128x96 @ 0.25 cm, 400 beamlets 0.5x0.5 cm2 at gantry 0, histories "
                                "log-uniform [1e4, 1e6] per beamlet, nbatch 10, sparse Dij at 1% of beamlet max")
             out["beamlets"] = res

@@ -38,20 +38,58 @@ __device__ __forceinline__ void philox10(uint32_t c0, uint32_t c1, uint32_t c2, 
     out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
 }
 
+// Look-ahead of a lane's stream in LDS: RNG_RING Philox blocks per lane,
+// block b in slot b % RNG_RING of the lane's column.  A transport kernel
+// that uses it tops the ring up where the wavefront's live lanes are
+// converged (the head of each trip), so the draw sites read 16 bytes from
+// LDS instead of running Philox in whichever few lanes crossed a block
+// boundary there.  The words and their order are those of the stream
+// itself.  The ring is the launch's dynamic LDS (RNG_RING_BYTES), so only
+// the launches that top it up pay for it in occupancy.
+constexpr uint32_t RNG_RING = 8;
+constexpr int RNG_LANES = 256;           // threads per transport block
+constexpr size_t RNG_RING_BYTES = (size_t)RNG_RING * RNG_LANES * sizeof(uint4);
+extern __shared__ uint4 rng_ring[];      // [RNG_RING][RNG_LANES]
+
 struct Rng {
     uint32_t k0, k1, ctr;
     uint32_t blk;            // block held in w[] (0xffffffff = none)
     uint32_t w0, w1, w2, w3;
+    uint32_t tail;           // blocks in [ctr >> 2, tail) are in this thread's LDS ring
 
-    __device__ __forceinline__ void reset_cache() { blk = 0xffffffffu; }
+    // No cached block, nothing in the ring: every block is computed at the
+    // draw (secondaries, fresh primaries, particles read from a queue).
+    // Only the stream of the particle a transport lane owns is topped up,
+    // so only that stream ever finds blocks in the thread's ring column.
+    __device__ __forceinline__ void reset_cache() { blk = 0xffffffffu; tail = 0u; }
+
+    // Fill the thread's ring up to `depth` blocks ahead of the next draw
+    // (transport kernels, for the stream of the lane's own particle).
+    __device__ __forceinline__ void top_up(uint32_t depth)
+    {
+        const uint32_t cur = ctr >> 2;
+        if (tail < cur) tail = cur;
+        while (tail < cur + depth) {
+            uint32_t o[4];
+            philox10(tail, 0u, 0u, 0u, k0, k1, o);
+            rng_ring[(tail % RNG_RING) * RNG_LANES + threadIdx.x] = make_uint4(o[0], o[1], o[2], o[3]);
+            tail++;
+        }
+    }
 
     __device__ __forceinline__ uint32_t next_u24()
     {
         uint32_t b = ctr >> 2;
         if (b != blk) {
-            uint32_t o[4];
-            philox10(b, 0u, 0u, 0u, k0, k1, o);
-            w0 = o[0]; w1 = o[1]; w2 = o[2]; w3 = o[3];
+            if (b < tail) {
+                const uint4 v = rng_ring[(b % RNG_RING) * RNG_LANES + threadIdx.x];
+                w0 = v.x; w1 = v.y; w2 = v.z; w3 = v.w;
+            } else {
+                uint32_t o[4];
+                philox10(b, 0u, 0u, 0u, k0, k1, o);
+                w0 = o[0]; w1 = o[1]; w2 = o[2]; w3 = o[3];
+                tail = b + 1;
+            }
             blk = b;
         }
         // word (ctr & 3) of the block by two selects on its bits (a select

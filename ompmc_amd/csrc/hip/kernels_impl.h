@@ -131,13 +131,13 @@ void Launch<R>::source(const DevJob *jobs, const DevPhantom<R> *ph, const Source
 template <typename R>
 void Launch<R>::photon(const KArgs<R> &a, int grid, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_photon<R>, dim3(grid), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_photon<R>, dim3(grid), dim3(RNG_LANES), OMC_RNG_DEPTH_P ? RNG_RING_BYTES : 0, s, a);
 }
 
 template <typename R>
 void Launch<R>::electron(const KArgs<R> &a, int grid, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_electron<R>, dim3(grid), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(k_electron<R>, dim3(grid), dim3(RNG_LANES), OMC_RNG_DEPTH_E ? RNG_RING_BYTES : 0, s, a);
 }
 
 }  // namespace omc
