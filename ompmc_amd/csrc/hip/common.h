@@ -10,6 +10,13 @@
 
 namespace omc {
 
+// Global-memory view of a pointer.  A pointer read from LDS or from memory
+// carries no address space, so the compiler reaches it with FLAT
+// instructions, which also wait on the LDS counter; the cast makes the loads
+// and atomics through it global ones.
+template <typename T> using gmem = __attribute__((address_space(1))) T;
+template <typename T> __device__ __forceinline__ gmem<T> *glob(T *p) { return (gmem<T> *)p; }
+
 constexpr double RM = OMC_RM;
 constexpr int MXGE = OMC_MXGE;
 constexpr int MXEKE = OMC_MXEKE;

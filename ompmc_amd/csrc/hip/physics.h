@@ -76,7 +76,11 @@ __device__ __forceinline__ void voxel_of(int irl, int imax, int ijmax, int &irx,
     iry = small_div(rem, imax);
     irx = rem - iry * imax;
 }
-// howfar (omc_dosxyz.c:187-297) for a particle inside region irl > 0.
+// howfar (omc_dosxyz.c:187-297) for a particle inside region irl > 0.  The
+// boundary each axis heads for is loaded up front (the three loads are in
+// flight together; an axis with zero direction cosine loads one it does not
+// use, always inside the array), then the axes are tested in the
+// reference's order z, x, y.
 template <typename R, typename Geo>
 __device__ __forceinline__ void howfar(const Geo &g, int irl, R x, R y, R z, R u, R v, R w,
                                        int &irnew, R &ustep)
@@ -84,26 +88,29 @@ __device__ __forceinline__ void howfar(const Geo &g, int irl, R x, R y, R z, R u
     int ijmax = g.isize * g.jsize, imax = g.isize;
     int irx, iry, irz;
     voxel_of(irl, imax, ijmax, irx, iry, irz);
+    const R bz = (R)glob(g.zb)[irz + (w > (R)0 ? 1 : 0)];
+    const R bx = (R)glob(g.xb)[irx + (u > (R)0 ? 1 : 0)];
+    const R by = (R)glob(g.yb)[iry + (v > (R)0 ? 1 : 0)];
     R dist;
     if (w > (R)0) {
-        dist = ((R)g.zb[irz + 1] - z) / w;
+        dist = (bz - z) / w;
         if (dist < ustep) { ustep = dist; irnew = (irz != g.ksize - 1) ? irl + ijmax : 0; }
     } else if (w < (R)0) {
-        dist = -(z - (R)g.zb[irz]) / w;
+        dist = -(z - bz) / w;
         if (dist < ustep) { ustep = dist; irnew = (irz != 0) ? irl - ijmax : 0; }
     }
     if (u > (R)0) {
-        dist = ((R)g.xb[irx + 1] - x) / u;
+        dist = (bx - x) / u;
         if (dist < ustep) { ustep = dist; irnew = (irx != g.isize - 1) ? irl + 1 : 0; }
     } else if (u < (R)0) {
-        dist = -(x - (R)g.xb[irx]) / u;
+        dist = -(x - bx) / u;
         if (dist < ustep) { ustep = dist; irnew = (irx != 0) ? irl - 1 : 0; }
     }
     if (v > (R)0) {
-        dist = ((R)g.yb[iry + 1] - y) / v;
+        dist = (by - y) / v;
         if (dist < ustep) { ustep = dist; irnew = (iry != g.jsize - 1) ? irl + imax : 0; }
     } else if (v < (R)0) {
-        dist = -(y - (R)g.yb[iry]) / v;
+        dist = -(y - by) / v;
         if (dist < ustep) { ustep = dist; irnew = (iry != 0) ? irl - imax : 0; }
     }
 }
@@ -116,13 +123,14 @@ __device__ __forceinline__ R hownear(const Geo &g, int irl, R x, R y, R z)
     int ijmax = g.isize * g.jsize, imax = g.isize;
     int irx, iry, irz;
     voxel_of(irl, imax, ijmax, irx, iry, irz);
+    const gmem<const double> *xb = glob(g.xb), *yb = glob(g.yb), *zb = glob(g.zb);
     R t = (R)1.0E10;
-    t = m_fmin(t, (R)g.xb[irx + 1] - x);
-    t = m_fmin(t, x - (R)g.xb[irx]);
-    t = m_fmin(t, (R)g.yb[iry + 1] - y);
-    t = m_fmin(t, y - (R)g.yb[iry]);
-    t = m_fmin(t, (R)g.zb[irz + 1] - z);
-    t = m_fmin(t, z - (R)g.zb[irz]);
+    t = m_fmin(t, (R)xb[irx + 1] - x);
+    t = m_fmin(t, x - (R)xb[irx]);
+    t = m_fmin(t, (R)yb[iry + 1] - y);
+    t = m_fmin(t, y - (R)yb[iry]);
+    t = m_fmin(t, (R)zb[irz + 1] - z);
+    t = m_fmin(t, z - (R)zb[irz]);
     return t;
 }
 

@@ -42,18 +42,19 @@ template <typename R> struct KArgs {
 // pointer to its record instead of the record's fields.
 template <typename R> struct LaneGeo {
     const DevPhantom<R> *g;  // LDS copy of the particle's phantom record
-    double *tally;           // job tally row of this history's batch
-    double *ledger;          // job ledger entry of this history (or null)
+    gmem<double> *tally;     // job tally row of this history's batch
+    gmem<double> *ledger;    // job ledger entry of this history (or null)
 
     __device__ __forceinline__ void bind(const KArgs<R> &A, const DevPhantom<R> *lds, const Particle<R> &p)
     {
         const DevJob &J = A.jobs[p.job];
         g = &lds[J.phantom];
         uint32_t batch = (uint32_t)((uint64_t)p.hist / (uint64_t)J.nperbatch);
-        tally = J.tally + (size_t)batch * (size_t)g->nreg;
-        ledger = J.ledger ? J.ledger + 3 * (size_t)p.hist : nullptr;
+        tally = glob(J.tally) + (size_t)batch * (size_t)g->nreg;
+        ledger = J.ledger ? glob(J.ledger) + 3 * (size_t)p.hist : nullptr;
     }
-    __device__ __forceinline__ int medium(int ir) const { int m = g->med[ir]; return m == 0xff ? -1 : m; }
+    __device__ __forceinline__ int medium(int ir) const { int m = glob(g->med)[ir]; return m == 0xff ? -1 : m; }
+    __device__ __forceinline__ R rho(int ir) const { return glob(g->rhof)[ir]; }
 };
 
 // Copies the resident phantom records to LDS (all threads of the block).
@@ -77,8 +78,8 @@ template <typename R> struct Scorer {
     __device__ __forceinline__ void flush(const LaneGeo<R> &G)
     {
         if (ir >= 0) {
-            atomicAdd(&G.tally[ir], acc);
-            if (G.ledger) atomicAdd(&G.ledger[ir > 0 ? 0 : 1], acc);
+            atomicAdd((double *)(G.tally + ir), acc);
+            if (G.ledger) atomicAdd((double *)(G.ledger + (ir > 0 ? 0 : 1)), acc);
         }
         ir = -1;
         acc = 0.0;
@@ -138,9 +139,21 @@ __device__ __forceinline__ void yield_push(const Queue<R> &q, const Particle<R> 
 
 // ============================================================ photons ======
 template <typename R> struct PhotonLane {
-    int phase;               // 0: (re)enter photon(), 1: transport step
+    int phase;               // 0: (re)enter photon(), 1: transport step, 2: interaction
     int imed, lgle;
     R gle, dpmfp, gmfp, cohfac, eig;
+    // Not part of the saved state: the density of the current voxel and the
+    // table values of medium cmed at energy gle (gmfpr0, cohfac, lgle),
+    // reused while the photon keeps its energy and medium (-2 = none).
+    R rhof, gmfpr0;
+    int cmed;
+
+    // After the particle's voxel or energy was set from outside the step.
+    __device__ __forceinline__ void attach(const LaneGeo<R> &G, const Particle<R> &p)
+    {
+        rhof = G.rho(p.ir);
+        cmed = -2;
+    }
 
     __device__ __forceinline__ void save(const Queue<R> &q, uint32_t i) const
     {
@@ -201,6 +214,7 @@ __device__ __forceinline__ bool photon_event(const KArgs<R> &A, Particle<R> &p, 
     // photon() entry
     int irl = p.ir;
     s.imed = G.medium(irl);
+    s.attach(G, p);
     s.eig = p.e;
     if (p.e <= (R)G.g->pcut[s.imed + 1] || p.wt == (R)0) {
         sc.add(G, irl, (double)p.wt * (double)p.e);
@@ -230,13 +244,15 @@ __device__ __forceinline__ bool photon_step(const KArgs<R> &A, Particle<R> &p, P
     int irl = p.ir;
     R tstep;
     if (s.imed != -1) {
-        const MediumScalars &M = T.small->med[s.imed];
-        s.lgle = pwlf_bin(s.gle, M.ge1, M.ge0, MXGE);
-        R4<R> c = T.ph_mfp[s.imed * MXGE + s.lgle];
-        R gmfpr0 = s.gle * c.a1 + c.a0;
-        R rhof = G.g->rhof[irl];
-        s.gmfp = gmfpr0 / rhof;
-        s.cohfac = s.gle * c.b1 + c.b0;
+        if (s.imed != s.cmed) {
+            const MediumScalars &M = T.small->med[s.imed];
+            s.lgle = pwlf_bin(s.gle, M.ge1, M.ge0, MXGE);
+            R4<R> c = T.ph_mfp[s.imed * MXGE + s.lgle];
+            s.gmfpr0 = s.gle * c.a1 + c.a0;
+            s.cohfac = s.gle * c.b1 + c.b0;
+            s.cmed = s.imed;
+        }
+        s.gmfp = s.gmfpr0 / s.rhof;
         s.gmfp *= s.cohfac;
         tstep = s.gmfp * s.dpmfp;
     } else {
@@ -245,7 +261,7 @@ __device__ __forceinline__ bool photon_step(const KArgs<R> &A, Particle<R> &p, P
     int irnew = irl;
     R ustep = tstep;
     if (irl == 0) {                        // left the phantom: discard
-        if (G.ledger) atomicAdd(&G.ledger[2], (double)p.wt * (double)p.e);
+        if (G.ledger) atomicAdd((double *)(G.ledger + 2), (double)p.wt * (double)p.e);
         return false;
     }
     howfar<R>(*G.g, irl, p.x, p.y, p.z, p.u, p.v, p.w, irnew, ustep);
@@ -257,6 +273,7 @@ __device__ __forceinline__ bool photon_step(const KArgs<R> &A, Particle<R> &p, P
         ncross++;
         p.ir = irnew;
         s.imed = G.medium(irnew);
+        s.rhof = G.rho(irnew);
     }
     if (s.imed != -1 && s.dpmfp <= (R)SGMFP) s.phase = 2;
     return true;
@@ -383,7 +400,7 @@ __device__ __forceinline__ int zero_ustep(const KArgs<R> &A, Particle<R> &p, Ele
     int irl = p.ir;
     const int imed = s.imed;
     const MediumScalars *M = &T.small->med[imed];
-    const R rhof = G.g->rhof[irl];
+    const R rhof = s.rhof;
     const R tstep = (R)(0.0 / (double)rhof);
     const ElecBin<R> &eb = T.el[(imed * 2 + qel) * MXEKE + s.lelke];
     const ElecCsda &cs = T.csda[(imed * 2 + qel) * MXEKE + s.lelke];
@@ -407,7 +424,6 @@ __device__ __forceinline__ int zero_ustep(const KArgs<R> &A, Particle<R> &p, Ele
     tustep = m_fmin(tustep, m_fmax(tperp, skindepth));
     // from here on the general iteration's path is fixed
     nsteps++;
-    s.rhof = rhof;
     R rnno = urand<R>(p.rng);
     if (rnno < (R)1.0E-30) rnno = (R)1.0E-30;
     R lambda = (R)(-1.0) * m_log((R)1.0 - rnno);
@@ -426,6 +442,7 @@ __device__ __forceinline__ int zero_ustep(const KArgs<R> &A, Particle<R> &p, Ele
         p.ir = irnew;
         irl = irnew;
         s.imed = G.medium(irl);
+        s.rhof = G.rho(irl);
     }
     if (eie <= (R)G.g->ecut[s.imed + 1]) { electron_cutoff<R>(A, p, G, sc, eie); return E_DONE; }
     s.phase = 1;
@@ -445,7 +462,7 @@ __device__ __forceinline__ int electron_iteration(const KArgs<R> &A, Particle<R>
     if (s.phase == 0) {
         int irl = p.ir;
         s.imed = G.medium(irl);
-        s.rhof = G.g->rhof[irl];
+        s.rhof = G.rho(irl);
         s.medold = s.imed;
         if (eie <= (R)G.g->ecut[s.imed + 1]) { electron_cutoff<R>(A, p, G, sc, eie); return E_DONE; }
         s.phase = 1;
@@ -491,8 +508,7 @@ __device__ __forceinline__ int electron_iteration(const KArgs<R> &A, Particle<R>
     } else {
         csb = T.csda + (imed * 2 + qel) * MXEKE;
         const double eke0 = M->eke0, eke1 = M->eke1;
-        s.rhof = G.g->rhof[irl];
-        const R rhof = s.rhof;
+        const R rhof = s.rhof;          // density of the current voxel, loaded on entering it
         if (s.sig0 <= (R)0.0) {
             tstep = (R)10.0E8;
             s.sig0 = (R)1.0E-15;
@@ -613,6 +629,7 @@ __device__ __forceinline__ int electron_iteration(const KArgs<R> &A, Particle<R>
             p.ir = irnew;
             irl = irnew;
             s.imed = G.medium(irl);
+            s.rhof = G.rho(irl);
         }
         if (eie <= (R)G.g->ecut[s.imed + 1]) { electron_cutoff<R>(A, p, G, sc, eie); return E_DONE; }
         // leave the ustep loop; the tstep loop restarts unless a step through
@@ -684,6 +701,7 @@ __device__ __forceinline__ int electron_iteration(const KArgs<R> &A, Particle<R>
         p.ir = irnew;
         irl = irnew;
         s.imed = G.medium(irl);
+        s.rhof = G.rho(irl);
     }
     if (eie <= (R)G.g->ecut[s.imed + 1]) { electron_cutoff<R>(A, p, G, sc, eie); return E_DONE; }
     if (s.imed != s.medold) { s.phase = 1; return E_ALIVE; }
@@ -732,6 +750,12 @@ __device__ __forceinline__ int electron_iteration(const KArgs<R> &A, Particle<R>
 #endif
 static_assert(OMC_RNG_DEPTH_E <= RNG_RING && OMC_RNG_DEPTH_P <= RNG_RING, "ring depth");
 
+// Free lanes of a wavefront wait for a new particle until at least this many
+// are free (or no lane is live).
+#ifndef OMC_FETCH_MIN
+#define OMC_FETCH_MIN 16
+#endif
+
 template <typename R> struct Waves;
 template <> struct Waves<float> { static constexpr int photon = OMC_WAVES_P32, electron = OMC_WAVES_E32; };
 template <> struct Waves<double> { static constexpr int photon = OMC_WAVES_P64, electron = OMC_WAVES_E64; };
@@ -749,7 +773,11 @@ __global__ void __launch_bounds__(256, Waves<R>::photon) k_photon(KArgs<R> A)
     uint32_t nsteps = 0, ncross = 0, trips = 0, here = 0, nyield = 0;
     const uint32_t in_end = min(*A.in_count, A.qin.cap);
     for (;;) {
-        if (!alive && !exhausted) {
+        // free lanes take new particles together (one counter atomic and one
+        // round of queue loads for all of them), once enough are free or
+        // none is live
+        const unsigned long long want = __ballot(!alive && !exhausted);
+        if (!alive && !exhausted && ((uint32_t)__popcll(want) >= OMC_FETCH_MIN || __ballot(alive) == 0)) {
             uint32_t i = wave_reserve(A.fetch, 1);
             if (i < in_end) {
                 if (q_load(A.qin, i, p)) {
@@ -760,6 +788,7 @@ __global__ void __launch_bounds__(256, Waves<R>::photon) k_photon(KArgs<R> A)
                     trips = 0;
                 }
                 G.bind(A, lds_ph, p);
+                s.attach(G, p);
                 here = 0;
                 alive = true;
             } else {
@@ -814,7 +843,11 @@ __global__ void __launch_bounds__(256, Waves<R>::electron) k_electron(KArgs<R> A
     uint32_t nsteps = 0, ncross = 0, trips = 0, here = 0, nyield = 0;
     const uint32_t in_end = min(*A.in_count, A.qin.cap);
     for (;;) {
-        if (!alive && !exhausted) {
+        // free lanes take new particles together (one counter atomic and one
+        // round of queue loads for all of them), once enough are free or
+        // none is live
+        const unsigned long long want = __ballot(!alive && !exhausted);
+        if (!alive && !exhausted && ((uint32_t)__popcll(want) >= OMC_FETCH_MIN || __ballot(alive) == 0)) {
             uint32_t i = wave_reserve(A.fetch, 1);
             if (i < in_end) {
                 if (q_load(A.qin, i, p)) {
