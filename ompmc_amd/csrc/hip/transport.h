@@ -232,50 +232,119 @@ __device__ __forceinline__ bool photon_event(const KArgs<R> &A, Particle<R> &p, 
     return true;
 }
 
-// One trip of the transport loop (ompmc.c:1951-2019); a step that ends in
-// an interaction leaves the photon in phase 2.  Returns true while the
-// photon is still owned by the lane.
+// Voxel crossings a photon trip traces ahead and runs.  On the slab phantom
+// FP32 trips of 1 / 4 / 8 / 12 / 16 crossings gave 61 / 74 / 80 / 81 / 81 M
+// histories/s (16 spills registers); FP64 fits 12 without spills.
+#ifndef OMC_PHOTON_STEPS_32
+#define OMC_PHOTON_STEPS_32 12
+#endif
+#ifndef OMC_PHOTON_STEPS_64
+#define OMC_PHOTON_STEPS_64 12
+#endif
+template <typename R> struct PhotonSteps;
+template <> struct PhotonSteps<float> { static constexpr int value = OMC_PHOTON_STEPS_32; };
+template <> struct PhotonSteps<double> { static constexpr int value = OMC_PHOTON_STEPS_64; };
+
+// One trip of photon transport: up to PhotonSteps<R> iterations of the
+// reference's transport loop (ompmc.c:1951-2019).  A straight run of voxel
+// crossings needs the medium and density of each voxel entered, and fetching
+// them one crossing at a time puts a memory round trip into every
+// iteration.  So the trip first traces the straight path ahead through up to
+// PhotonSteps<R> voxel boundaries with howfar (with no step limit the axis
+// it picks is the one the step picks whenever the step reaches a boundary)
+// and moves a copy of the position as the steps will, then loads all media
+// and densities at once, then runs the iterations on them.  It stops at the
+// first iteration that ends inside its voxel: an interaction (phase 2), or a
+// step cut short by rounding, after which the traced path no longer applies.
+// Returns true while the photon is still owned by the lane.
+template <typename R>
+__device__ __forceinline__ void advance(R &x, R &y, R &z, R u, R v, R w, R ustep)
+{
+    x += ustep * u;
+    y += ustep * v;
+    z += ustep * w;
+}
+
 template <typename R>
 __device__ __forceinline__ bool photon_step(const KArgs<R> &A, Particle<R> &p, PhotonLane<R> &s, LaneGeo<R> &G,
                                             uint32_t &nsteps, uint32_t &ncross)
 {
     const DevTables<R> &T = A.T;
-    nsteps++;
-    int irl = p.ir;
-    R tstep;
-    if (s.imed != -1) {
-        if (s.imed != s.cmed) {
-            const MediumScalars &M = T.small->med[s.imed];
-            s.lgle = pwlf_bin(s.gle, M.ge1, M.ge0, MXGE);
-            R4<R> c = T.ph_mfp[s.imed * MXGE + s.lgle];
-            s.gmfpr0 = s.gle * c.a1 + c.a0;
-            s.cohfac = s.gle * c.b1 + c.b0;
-            s.cmed = s.imed;
+    constexpr int K = PhotonSteps<R>::value;
+    int irk[K], medk[K];
+    R dk[K], rhok[K];
+    int n = 0;
+    {
+        int irc = p.ir;
+        R x = p.x, y = p.y, z = p.z;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            if (irc != 0 && n == k) {
+                int irn = irc;
+                R us = (R)3.0E38;
+                howfar<R>(*G.g, irc, x, y, z, p.u, p.v, p.w, irn, us);
+                advance<R>(x, y, z, p.u, p.v, p.w, us);
+                irk[k] = irn;
+                dk[k] = us;
+                irc = irn;
+                n = k + 1;
+            }
         }
-        s.gmfp = s.gmfpr0 / s.rhof;
-        s.gmfp *= s.cohfac;
-        tstep = s.gmfp * s.dpmfp;
-    } else {
-        tstep = (R)1.0E8;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            if (k < n) {
+                medk[k] = G.medium(irk[k]);
+                rhok[k] = G.rho(irk[k]);
+            }
+        }
     }
-    int irnew = irl;
-    R ustep = tstep;
-    if (irl == 0) {                        // left the phantom: discard
-        if (G.ledger) atomicAdd((double *)(G.ledger + 2), (double)p.wt * (double)p.e);
-        return false;
-    }
-    howfar<R>(*G.g, irl, p.x, p.y, p.z, p.u, p.v, p.w, irnew, ustep);
-    p.x += ustep * p.u;
-    p.y += ustep * p.v;
-    p.z += ustep * p.w;
-    if (s.imed != -1) s.dpmfp = m_fmax((R)0.0, s.dpmfp - ustep / s.gmfp);
-    if (irnew != irl) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        if (k > 0 && k >= n) break;
+        nsteps++;
+        const int irl = p.ir;
+        R tstep;
+        if (s.imed != -1) {
+            if (s.imed != s.cmed) {
+                const MediumScalars &M = T.small->med[s.imed];
+                s.lgle = pwlf_bin(s.gle, M.ge1, M.ge0, MXGE);
+                R4<R> c = T.ph_mfp[s.imed * MXGE + s.lgle];
+                s.gmfpr0 = s.gle * c.a1 + c.a0;
+                s.cohfac = s.gle * c.b1 + c.b0;
+                s.cmed = s.imed;
+            }
+            s.gmfp = s.gmfpr0 / s.rhof;
+            s.gmfp *= s.cohfac;
+            tstep = s.gmfp * s.dpmfp;
+        } else {
+            tstep = (R)1.0E8;
+        }
+        if (irl == 0) {                    // left the phantom: discard
+            if (G.ledger) atomicAdd((double *)(G.ledger + 2), (double)p.wt * (double)p.e);
+            return false;
+        }
+        // howfar with the step limit tstep: the traced boundary if nearer
+        R ustep = tstep;
+        int irnew = irl;
+        if (dk[k] < ustep) {
+            ustep = dk[k];
+            irnew = irk[k];
+        }
+        advance<R>(p.x, p.y, p.z, p.u, p.v, p.w, ustep);
+        if (s.imed != -1) s.dpmfp = m_fmax((R)0.0, s.dpmfp - ustep / s.gmfp);
+        if (irnew == irl) {
+            if (s.imed != -1 && s.dpmfp <= (R)SGMFP) s.phase = 2;
+            return true;
+        }
         ncross++;
         p.ir = irnew;
-        s.imed = G.medium(irnew);
-        s.rhof = G.rho(irnew);
+        s.imed = medk[k];
+        s.rhof = rhok[k];
+        if (s.imed != -1 && s.dpmfp <= (R)SGMFP) {
+            s.phase = 2;
+            return true;
+        }
     }
-    if (s.imed != -1 && s.dpmfp <= (R)SGMFP) s.phase = 2;
     return true;
 }
 
