@@ -171,9 +171,11 @@ struct ElecCsda {
 };
 
 // multiple-scattering surface entry: alias threshold, alias index, u node
-struct MsEntry {
-    double ums, wms;
-    int ims, pad;
+// One node of a multiple-scattering alias table (mscatSample tables): the
+// alias weight, the node's value and the value of the node it aliases
+// (same row), so that a draw is one load.
+template <typename R> struct MsAlias {
+    R wms, ums, ualias, pad;
 };
 
 // Per-medium scalars and the small tables.
@@ -204,7 +206,7 @@ template <typename R> struct DevTables {
     const ElecBin<R> *el;    // [(imed*2 + qel)*MXEKE + bin]
     const ElecCsda *csda;    // [(imed*2 + qel)*MXEKE + bin]
     const R *spin;           // [imed*32768 + qel*16384 + i*512 + j*32 + k]
-    const MsEntry *ms;       // [i*256 + j*32 + k]
+    const MsAlias<R> *ms;    // [i*256 + j*32 + k]
 };
 
 // Resident phantom: region r = voxel + 1, region 0 = outside.

@@ -376,8 +376,12 @@ template <typename R> static int upload_tables(omc_dev_ctx *c, const omc_tables 
             }
     std::vector<R> spin((size_t)nmed * OMC_SPIN_PER_MED);
     for (size_t i = 0; i < spin.size(); ++i) spin[i] = (R)t->spin_rej[i];
-    std::vector<MsEntry> ms(OMC_MS_SIZE);
-    for (int i = 0; i < OMC_MS_SIZE; ++i) ms[i] = { t->ums[i], t->wms[i], t->ims[i], 0 };
+    std::vector<MsAlias<R>> ms(OMC_MS_SIZE);
+    for (int i = 0; i < OMC_MS_SIZE; ++i) {
+        const int row = i - i % 32, a = t->ims[i];
+        if (a < 0 || a >= 32) return fail(OMC_EFORMAT, "multiple-scattering alias index %d out of its row", a);
+        ms[i] = { (R)t->wms[i], (R)t->ums[i], (R)t->ums[row + a], (R)0 };
+    }
 
     struct { void **dst; const void *src; size_t bytes; } up[] = {
         { &c->small, &S, sizeof S },
@@ -409,7 +413,7 @@ template <typename R> static DevTables<R> tables_view(omc_dev_ctx *c)
     T.el = (const ElecBin<R> *)c->el;
     T.csda = (const ElecCsda *)c->csda;
     T.spin = (const R *)c->spin;
-    T.ms = (const MsEntry *)c->ms;
+    T.ms = (const MsAlias<R> *)c->ms;
     return T;
 }
 

@@ -218,17 +218,17 @@ __device__ void mscat(const DevTables<R> &T, Rng &rng, int imed, int qel, bool &
                          ((R)-2.77164 + llmbda * ((R)2.94874 - llmbda * ((R)0.1535754 - llmbda * (R)0.00552888)));
             find_index = false;
         }
-        const MsEntry *row = T.ms + mi.i * 256 + mi.j * 32;
+        const MsAlias<R> *row = T.ms + mi.i * 256 + mi.j * 32;
         do {
             xi = urand<R>(rng);
             R ak = xi * (R)31;
             int k = (int)ak;
             ak -= (R)k;
-            MsEntry me = row[k];
-            if (ak > (R)me.wms) me = row[me.ims];
-            // the reference's interpolation step has du = ums[k] - ums[k] = 0,
-            // so u is the node value; its random number is still drawn
-            R u = (R)me.ums;
+            const MsAlias<R> me = row[k];
+            // node k, or the node it aliases when ak is above its weight; the
+            // reference's interpolation step has du = ums[k] - ums[k] = 0, so
+            // u is the node value; its random number is still drawn
+            R u = ak > me.wms ? me.ualias : me.ums;
             (void)urand<R>(rng);
             xi = omega2 * u / ((R)1.0 + (R)0.5 * omega2 - u);
             if (xi > (R)1.99999) xi = (R)1.99999;
