@@ -45,7 +45,7 @@ import numpy as np
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
-# The engine drives 4 pipeline streams plus a helper stream; with HIP's
+# The engine drives 6 pipeline streams plus a helper stream; with HIP's
 # default of 4 hardware queues per process two of them can share a queue and
 # serialise.  Give the process 8 before the HIP runtime starts.
 if int(os.environ.get("GPU_MAX_HW_QUEUES", "0") or 0) < 8:

@@ -63,12 +63,15 @@ struct DevBuf {
 constexpr int MAX_JOBS = 4096;
 constexpr int MAX_SEGS = 512;
 constexpr int MAX_PIPES = 8;
-constexpr int DEFAULT_PIPES = 4;
-// Defaults from a sweep of pipelines x chunk x yield budget on the 64^3 slab
-// phantom with the 6 MV spectrum (16M histories, FP32 and FP64): throughput
-// grows with the particles per launch (lanes then refill many times before
-// the launch drains) and with overlapping pipelines.
-constexpr int64_t DEFAULT_CHUNK = 1 << 21;
+constexpr int DEFAULT_PIPES = 6;
+// Defaults from sweeps of pipelines x chunk on the 64^3 slab phantom with
+// the 6 MV spectrum (20M histories, FP32 and FP64): throughput grows with the
+// particles per launch (lanes refill many times before the launch drains)
+// and with overlapping pipelines up to 6 (FP32 histories/s: 4 x 2M 100M,
+// 4 x 4M 106M, 6 x 4M 111-113M, 8 x 4M 108M; FP64 72M / 76M / 74-76M / 72M).
+// 6 pipeline streams and the helper stream fit the 8 hardware queues the
+// drivers ask for; 8 pipelines would share queues there.
+constexpr int64_t DEFAULT_CHUNK = 1 << 22;
 constexpr uint32_t DEFAULT_YIELD_TRIPS = 4096;
 // Drain compaction: once a launch's input is used up, a wavefront with at
 // most 32 live lanes hands its particles (after 16 trips in the launch) to

@@ -18,7 +18,7 @@ double drv_time(void)
 
 void drv_parse_args(drv_args *a, int argc, char **argv)
 {
-    /* the engine drives 4 pipeline streams plus a helper stream: give the
+    /* the engine drives 6 pipeline streams plus a helper stream: give the
      * process 8 hardware queues (HIP's default is 4) before HIP starts, so
      * that no two pipelines share one */
     const char *q = getenv("GPU_MAX_HW_QUEUES");
