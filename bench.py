@@ -22,6 +22,8 @@ Also reported:
   latency      Poisson stream of single-beamlet requests, each sized for 2%
                relative sigma at its Dmax voxel, served by continuous batching;
                p50/p99 of submit -> result, FP32 and FP64 at the same load
+  autotune     what omc_dev_tune picks for the FP32 engine on the slab
+               workload (probe = one step's histories) and the rate it measured
   cpu_baseline the reference's OpenMP omc_dosxyz on a bounded sample of the
                slab workload, on this host's cores
 
@@ -110,6 +112,22 @@ def time_slab(prob, precision, hist, steps, warmup, pipelines=0, chunk=0):
             etot.append(float(s[1:].sum()))
         el = time.perf_counter() - t0
     return el, stats, etot, nper * nbatch
+
+
+def autotune_section(prob, precision, probe=4_000_000):
+    """omc_dev_tune on this workload: the settings it picks (from the engine
+    defaults) and the histories/s it measured for them on the probe."""
+    from ompmc_amd import device as D
+    with D.Engine(precision) as eng:
+        eng.upload_media(prob.tab)
+        pid = eng.upload_phantom(prob.ph, prob.reg)
+        eng.run_dosxyz(pid, prob.src, 100000, 10, seed=(3, 3), nreg=prob.reg.nreg)      # warm-up
+        t0 = time.perf_counter()
+        best, hps = eng.tune(pid, prob.src, probe, nbatch=10, seed=(4, 4))
+        el = time.perf_counter() - t0
+    keys = ("pipelines", "chunk", "yield_trips", "drain_lanes", "drain_trips", "event_div", "fetch_min")
+    return {"probe_histories": probe, "seconds": el, "histories_per_s": hps,
+            "chosen": {k: best[k] for k in keys}}
 
 
 def profiled_traffic(kernel):
@@ -358,6 +376,7 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-beamlets", action="store_true")
     ap.add_argument("--no-latency", action="store_true")
+    ap.add_argument("--no-tune", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU per process; --gpus must be 1 "
@@ -396,6 +415,9 @@ def main():
         out["fp32_over_fp64"] = hps32 / hps64
         out["energy_per_history_MeV"] = {"fp32": float(np.mean(e32)) / nh, "fp64": float(np.mean(e64)) / nh}
     out["device_stats_fp32_last_step"] = st32[-1]
+    if not args.no_tune:
+        log("autotune FP32 on the slab workload")
+        out["autotune"] = autotune_section(prob, D.FP32, probe=args.hist)
     if not args.no_beamlets or not args.no_latency:
         pprob, src, bl_nh = patient_problem(work)
         if not args.no_beamlets:

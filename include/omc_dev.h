@@ -54,6 +54,12 @@ typedef struct {
                                 yield); 0 = automatic, < 0 = never */
     int     drain_trips;     /* ...after they have taken this many trips in
                                 the launch; 0 = automatic */
+    int     event_div;       /* a photon wavefront runs its waiting
+                                interactions once 1/event_div of its live
+                                lanes wait for one; 0 = automatic */
+    int     fetch_min;       /* free lanes of a wavefront take new particles
+                                together once this many are free;
+                                0 = automatic */
 } omc_dev_opts;
 
 /* Device counters of one run (all sums over the run). */
@@ -146,6 +152,22 @@ int  omc_dev_beamlets_sparse(omc_dev_ctx *ctx, const omc_source_beamlets *src,
                              const int64_t *nhist, const omc_dev_run_params *rp,
                              double rel_thr, int64_t *jcs, int64_t *irs,
                              double *vals, int64_t cap, int64_t *nnz);
+
+/* Autotuner for one workload (no counterpart in the reference, whose only
+ * schedule knob is the OpenMP thread count).  Times the omc_dosxyz batch loop
+ * of `src` on `probe` histories (rp->nbatch batches; rp->nperbatch, ledger
+ * and stats unused) under each setting of one option at a time, from the
+ * context's current settings (coordinate descent, one pass):
+ *   pipelines  2, 4, 6, 8 (at most GPU_MAX_HW_QUEUES - 1 when that is set)
+ *   chunk      1M, 2M, 4M, 8M live particles per pipeline (within the budget)
+ *   yield_trips 1024, 4096, 16384;  drain_lanes 16, 32, 48;  drain_trips 8, 16, 32
+ *   event_div  1, 2, 4;  fetch_min 4, 16, 32
+ * Each setting is the better of two runs.  The histories, and so the
+ * results, do not depend on any of these.  The context keeps the fastest
+ * settings; *best receives them (device, precision and budget as they were)
+ * and *hps the histories/s measured for them.  No work may be queued. */
+int  omc_dev_tune(omc_dev_ctx *ctx, const omc_source_dosxyz *src, const omc_dev_run_params *rp,
+                  int64_t probe, omc_dev_opts *best, double *hps);
 
 /* ---- serving: continuous batching of dose requests ----------------------- */
 typedef struct {

@@ -21,6 +21,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <map>
@@ -79,6 +80,12 @@ constexpr uint32_t DEFAULT_YIELD_TRIPS = 4096;
 // 38 -> 55M histories/s, FP64 37 -> 42M).
 constexpr uint32_t DEFAULT_DRAIN_LANES = 32;
 constexpr uint32_t DEFAULT_DRAIN_TRIPS = 16;
+// A photon wavefront runs its waiting events (interaction, re-entry) once
+// half its live lanes wait for one (slab sweep of 2/4/8/16: 2 first, 22%
+// more histories/s than running each event as it comes); free lanes take new
+// particles 16 at a time.
+constexpr uint32_t DEFAULT_EVENT_DIV = 2;
+constexpr uint32_t DEFAULT_FETCH_MIN = 16;
 
 struct PhantomSlot {
     bool used = false;
@@ -179,6 +186,7 @@ struct omc_dev_ctx {
     uint32_t qcap = 0;
     uint32_t yield_trips = 0;
     uint32_t drain_lanes = 0, drain_trips = 0;
+    uint32_t event_div = 0, fetch_min = 0;
     int cus = 256;
     std::chrono::steady_clock::time_point t0;
 
@@ -727,6 +735,8 @@ template <typename R> static int launch_generation(omc_dev_ctx *c, int pi)
     A.yield_trips = c->yield_trips;
     A.drain_lanes = c->drain_lanes;
     A.drain_trips = c->drain_trips;
+    A.event_div = c->event_div;
+    A.fetch_min = c->fetch_min;
     const int max_grid = c->cus * 8;
     P.had_ph = inP > 0;
     if (P.had_ph) {
@@ -956,6 +966,43 @@ extern "C" {
 
 const char *omc_dev_last_error(void) { return g_err; }
 
+// Device bytes one pipeline holds (mirrors alloc_pipe).
+static int64_t pipe_bytes(const omc_dev_ctx *c)
+{
+    const size_t pb = c->precision == OMC_PREC_FP64 ? particle_bytes<double>() : particle_bytes<float>();
+    return 4 * (int64_t)pb * c->qcap + 8 * sizeof(uint32_t) + sizeof(DevStatus) + MAX_JOBS * sizeof(uint32_t) +
+           MAX_SEGS * sizeof(SourceSeg);
+}
+
+// (Re)builds the pipelines: npipes streams, each with queues four times
+// `chunk` deep (a generation's secondaries); the queues take at most half
+// the budget, which can lower the chunk.  No work may be queued.
+static int configure_pipes(omc_dev_ctx *c, int npipes, int64_t chunk)
+{
+    for (auto &P : c->pipes) free_pipe(P);
+    if (!c->pipes.empty()) c->used -= (int64_t)c->pipes.size() * pipe_bytes(c);
+    c->pipes.clear();
+    npipes = std::max(1, std::min(npipes, MAX_PIPES));
+    const size_t pb = c->precision == OMC_PREC_FP64 ? particle_bytes<double>() : particle_bytes<float>();
+    c->chunk = chunk;
+    int64_t cap = std::max<int64_t>(4 * c->chunk, 1 << 12);
+    const int64_t cap_mem = c->budget / 2 / npipes / 4 / (int64_t)pb;
+    if (cap > cap_mem) {
+        cap = cap_mem;
+        c->chunk = std::max<int64_t>(cap / 4, 1);
+    }
+    cap = std::min<int64_t>(cap, 1 << 26);
+    c->chunk = std::min<int64_t>(c->chunk, cap / 4 > 0 ? cap / 4 : 1);
+    if (cap < 64) return fail(OMC_EBUDGET, "HBM budget %lld too small for the particle queues", (long long)c->budget);
+    c->qcap = (uint32_t)cap;
+    c->pipes.resize(npipes);
+    for (auto &P : c->pipes) {
+        const int rc = c->precision == OMC_PREC_FP64 ? alloc_pipe<double>(c, P) : alloc_pipe<float>(c, P);
+        if (rc) return rc;
+    }
+    return OMC_OK;
+}
+
 int omc_dev_create(omc_dev_ctx **out, const omc_dev_opts *opts)
 {
     if (!out) return fail(OMC_EARG, "null context pointer");
@@ -978,36 +1025,19 @@ int omc_dev_create(omc_dev_ctx **out, const omc_dev_opts *opts)
     c->yield_trips = o.yield_trips > 0 ? (uint32_t)o.yield_trips : DEFAULT_YIELD_TRIPS;
     c->drain_lanes = o.drain_lanes > 0 ? (uint32_t)std::min(o.drain_lanes, 64) : o.drain_lanes < 0 ? 0u : DEFAULT_DRAIN_LANES;
     c->drain_trips = o.drain_trips > 0 ? (uint32_t)o.drain_trips : DEFAULT_DRAIN_TRIPS;
+    c->event_div = o.event_div > 0 ? (uint32_t)o.event_div : DEFAULT_EVENT_DIV;
+    c->fetch_min = o.fetch_min > 0 ? (uint32_t)std::min(o.fetch_min, 64) : DEFAULT_FETCH_MIN;
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, o.device) == hipSuccess) c->cus = prop.multiProcessorCount;
     size_t freeb = 0, totalb = 0;
     (void)hipMemGetInfo(&freeb, &totalb);
     c->budget = o.hbm_budget > 0 ? o.hbm_budget : (int64_t)(0.9 * (double)freeb);
-    const int npipes = o.pipelines > 0 ? o.pipelines : DEFAULT_PIPES;
-    const size_t pb = o.precision == OMC_PREC_FP64 ? particle_bytes<double>() : particle_bytes<float>();
-    // live particles per pipeline, and queues four times that deep (a
-    // generation's secondaries); queues take at most half the budget
-    c->chunk = o.chunk > 0 ? o.chunk : DEFAULT_CHUNK;
-    int64_t cap = std::max<int64_t>(4 * c->chunk, 1 << 12);
-    const int64_t cap_mem = c->budget / 2 / npipes / 4 / (int64_t)pb;
-    if (cap > cap_mem) {
-        cap = cap_mem;
-        c->chunk = std::max<int64_t>(cap / 4, 1);
-    }
-    cap = std::min<int64_t>(cap, 1 << 26);
-    c->chunk = std::min<int64_t>(c->chunk, cap / 4 > 0 ? cap / 4 : 1);
     int rc = OMC_OK;
-    if (cap < 64) rc = fail(OMC_EBUDGET, "HBM budget %lld too small for the particle queues", (long long)c->budget);
-    c->qcap = (uint32_t)std::max<int64_t>(cap, 0);
-    if (!rc && hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking) != hipSuccess) rc = fail(OMC_EDEVICE, "stream");
+    if (hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking) != hipSuccess) rc = fail(OMC_EDEVICE, "stream");
     if (!rc) rc = dalloc(c, (void **)&c->d_jobs, MAX_JOBS * sizeof(DevJob));
     const size_t phb = o.precision == OMC_PREC_FP64 ? sizeof(DevPhantom<double>) : sizeof(DevPhantom<float>);
     if (!rc) rc = dalloc(c, &c->d_phantoms, MAX_PHANTOMS * phb);
-    c->pipes.resize(rc ? 0 : npipes);
-    for (auto &P : c->pipes) {
-        if (rc) break;
-        rc = o.precision == OMC_PREC_FP64 ? alloc_pipe<double>(c, P) : alloc_pipe<float>(c, P);
-    }
+    if (!rc) rc = configure_pipes(c, o.pipelines > 0 ? o.pipelines : DEFAULT_PIPES, o.chunk > 0 ? o.chunk : DEFAULT_CHUNK);
     if (rc) { omc_dev_destroy(c); return rc; }
     *out = c;
     return OMC_OK;
@@ -1160,6 +1190,116 @@ int omc_dev_run_dosxyz(omc_dev_ctx *c, const omc_source_dosxyz *src, const omc_d
     k.sum2 = sum2;
     k.ledger = rp->ledger;
     return run_sink(c, k, works, rp->stats);
+}
+
+// ------------------------------------------------------------- autotuner --
+namespace {
+struct TuneSetting {
+    int pipes;
+    int64_t chunk;
+    int yield_trips, drain_lanes, drain_trips, event_div, fetch_min;
+};
+
+TuneSetting current_setting(const omc_dev_ctx *c)
+{
+    return { (int)c->pipes.size(), c->chunk, (int)c->yield_trips, (int)c->drain_lanes, (int)c->drain_trips,
+             (int)c->event_div, (int)c->fetch_min };
+}
+
+int apply_setting(omc_dev_ctx *c, const TuneSetting &t)
+{
+    if (t.pipes != (int)c->pipes.size() || t.chunk != c->chunk) OMC_TRY_DEV(configure_pipes(c, t.pipes, t.chunk));
+    c->yield_trips = (uint32_t)t.yield_trips;
+    c->drain_lanes = (uint32_t)t.drain_lanes;
+    c->drain_trips = (uint32_t)t.drain_trips;
+    c->event_div = (uint32_t)t.event_div;
+    c->fetch_min = (uint32_t)t.fetch_min;
+    return OMC_OK;
+}
+
+}  // namespace
+
+int omc_dev_tune(omc_dev_ctx *c, const omc_source_dosxyz *src, const omc_dev_run_params *rp, int64_t probe,
+                 omc_dev_opts *best, double *hps)
+{
+    OMC_TRY_DEV(check_run(c, rp));
+    if (!src || !best || !hps) return fail(OMC_EARG, "null argument");
+    if (probe < rp->nbatch) return fail(OMC_EARG, "probe of %lld histories for %d batches", (long long)probe, rp->nbatch);
+    if (any_running(c) || !c->waiting.empty()) return fail(OMC_EARG, "the autotuner needs a context with no queued work");
+    HIPCHK(hipSetDevice(c->device));
+    int max_pipes = MAX_PIPES;
+    if (const char *q = getenv("GPU_MAX_HW_QUEUES")) {
+        const int n = atoi(q);
+        if (n > 1) max_pipes = std::min(max_pipes, n - 1);
+    }
+    omc_dev_run_params r = *rp;
+    r.nperbatch = probe / rp->nbatch;
+    r.ledger = nullptr;
+    r.stats = nullptr;
+    const int nreg = c->ph[rp->phantom_id].nreg;
+    std::vector<double> s1(nreg), s2(nreg);
+    auto measure = [&](const TuneSetting &t, double &v) -> int {
+        OMC_TRY_DEV(apply_setting(c, t));
+        v = 0;
+        for (int k = 0; k < 2; ++k) {
+            const auto t0 = std::chrono::steady_clock::now();
+            OMC_TRY_DEV(omc_dev_run_dosxyz(c, src, &r, s1.data(), s2.data()));
+            const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+            v = std::max(v, (double)(r.nperbatch * r.nbatch) / std::max(sec, 1e-9));
+        }
+        return OMC_OK;
+    };
+    // one option at a time, from the current settings (coordinate descent)
+    static const std::vector<int64_t> axes[7] = {
+        { 2, 4, 6, 8 },                              // pipelines
+        { 1 << 20, 1 << 21, 1 << 22, 1 << 23 },      // live particles per pipeline
+        { 1024, 4096, 16384 },                       // yield budget (trips)
+        { 16, 32, 48 },                              // drain lanes
+        { 8, 16, 32 },                               // drain trips
+        { 1, 2, 4 },                                 // photon event threshold divisor
+        { 4, 16, 32 },                               // fetch threshold (lanes)
+    };
+    TuneSetting bst = current_setting(c);
+    double bv = 0;
+    int rc = measure(bst, bv);
+    for (int a = 0; a < 7 && !rc; ++a) {
+        for (int64_t val : axes[a]) {
+            TuneSetting t = bst;
+            int *fields[7] = { &t.pipes, nullptr, &t.yield_trips, &t.drain_lanes, &t.drain_trips, &t.event_div,
+                               &t.fetch_min };
+            if (a == 1) {
+                if (t.chunk == val) continue;
+                t.chunk = val;
+            } else {
+                if (*fields[a] == (int)val) continue;
+                *fields[a] = (int)val;
+            }
+            if (t.pipes > max_pipes) continue;
+            double v = 0;
+            rc = measure(t, v);
+            if (rc) break;
+            if (v > bv) {
+                bv = v;
+                bst = t;
+            }
+        }
+    }
+    const int rc2 = apply_setting(c, bst);
+    if (rc) return rc;
+    OMC_TRY_DEV(rc2);
+    memset(best, 0, sizeof *best);
+    best->device = c->device;
+    best->precision = c->precision;
+    best->hbm_budget = c->budget;
+    best->chunk = c->chunk;
+    best->pipelines = (int)c->pipes.size();
+    best->yield_trips = (int)c->yield_trips;
+    best->drain_lanes = (int)c->drain_lanes;
+    best->drain_trips = (int)c->drain_trips;
+    best->event_div = (int)c->event_div;
+    best->fetch_min = (int)c->fetch_min;
+    *hps = bv;
+    return OMC_OK;
 }
 
 static int beamlet_works(omc_dev_ctx *c, const omc_source_beamlets *src, const int64_t *nhist,

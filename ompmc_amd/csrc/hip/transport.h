@@ -35,6 +35,8 @@ template <typename R> struct KArgs {
     uint32_t drain_lanes;    // once the input is used up, a wave with at most
                              // this many live lanes hands its particles on...
     uint32_t drain_trips;    // ...if they have taken this many trips here
+    uint32_t event_div;      // photon events run once 1/event_div of the live lanes wait
+    uint32_t fetch_min;      // free lanes take new particles once this many are free
 };
 
 // Per-lane geometry / scoring context for the particle the lane holds.  The
@@ -811,19 +813,7 @@ __device__ __forceinline__ int electron_iteration(const KArgs<R> &A, Particle<R>
 #ifndef OMC_RNG_DEPTH_P
 #define OMC_RNG_DEPTH_P 0
 #endif
-// A photon wavefront runs its waiting events once at least 1/DIV of its
-// live lanes wait for one.  On the slab phantom a sweep of 2, 4, 8, 16 put
-// 2 first, with 22% more histories/s than running each event as it comes.
-#ifndef OMC_PHOTON_EVENT_DIV
-#define OMC_PHOTON_EVENT_DIV 2
-#endif
 static_assert(OMC_RNG_DEPTH_E <= RNG_RING && OMC_RNG_DEPTH_P <= RNG_RING, "ring depth");
-
-// Free lanes of a wavefront wait for a new particle until at least this many
-// are free (or no lane is live).
-#ifndef OMC_FETCH_MIN
-#define OMC_FETCH_MIN 16
-#endif
 
 template <typename R> struct Waves;
 template <> struct Waves<float> { static constexpr int photon = OMC_WAVES_P32, electron = OMC_WAVES_E32; };
@@ -846,7 +836,7 @@ __global__ void __launch_bounds__(256, Waves<R>::photon) k_photon(KArgs<R> A)
         // round of queue loads for all of them), once enough are free or
         // none is live
         const unsigned long long want = __ballot(!alive && !exhausted);
-        if (!alive && !exhausted && ((uint32_t)__popcll(want) >= OMC_FETCH_MIN || __ballot(alive) == 0)) {
+        if (!alive && !exhausted && ((uint32_t)__popcll(want) >= A.fetch_min || __ballot(alive) == 0)) {
             uint32_t i = wave_reserve(A.fetch, 1);
             if (i < in_end) {
                 if (q_load(A.qin, i, p)) {
@@ -874,7 +864,7 @@ __global__ void __launch_bounds__(256, Waves<R>::photon) k_photon(KArgs<R> A)
         // wavefront together, once enough of the live lanes wait for one or
         // none can step; until then the waiting lanes sit out the steps
         const uint32_t nev = (uint32_t)__popcll(__ballot(alive && s.phase != 1));
-        const bool events = nev * OMC_PHOTON_EVENT_DIV >= nlive || nev == nlive;
+        const bool events = nev * A.event_div >= nlive || nev == nlive;
         if (alive && (s.phase == 1 || events)) {
             if (OMC_RNG_DEPTH_P) p.rng.top_up(OMC_RNG_DEPTH_P);
             alive = s.phase == 1 ? photon_step<R>(A, p, s, G, nsteps, ncross) : photon_event<R>(A, p, s, G, sc);
@@ -916,7 +906,7 @@ __global__ void __launch_bounds__(256, Waves<R>::electron) k_electron(KArgs<R> A
         // round of queue loads for all of them), once enough are free or
         // none is live
         const unsigned long long want = __ballot(!alive && !exhausted);
-        if (!alive && !exhausted && ((uint32_t)__popcll(want) >= OMC_FETCH_MIN || __ballot(alive) == 0)) {
+        if (!alive && !exhausted && ((uint32_t)__popcll(want) >= A.fetch_min || __ballot(alive) == 0)) {
             uint32_t i = wave_reserve(A.fetch, 1);
             if (i < in_end) {
                 if (q_load(A.qin, i, p)) {

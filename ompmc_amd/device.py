@@ -16,7 +16,10 @@ _dp = C.POINTER(C.c_double)
 class DevOpts(C.Structure):
     _fields_ = [("device", C.c_int), ("precision", C.c_int), ("hbm_budget", C.c_int64), ("chunk", C.c_int64),
                 ("pipelines", C.c_int), ("yield_trips", C.c_int), ("drain_lanes", C.c_int),
-                ("drain_trips", C.c_int)]
+                ("drain_trips", C.c_int), ("event_div", C.c_int), ("fetch_min", C.c_int)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
 
 
 class DevStats(C.Structure):
@@ -70,6 +73,8 @@ def lib():
                                              C.POINTER(C.c_int)]
         L.omc_dev_release_phantom.argtypes = [C.c_void_p, C.c_int]
         L.omc_dev_run_dosxyz.argtypes = [C.c_void_p, C.POINTER(N.SourceDosxyz), C.POINTER(RunParams), _dp, _dp]
+        L.omc_dev_tune.argtypes = [C.c_void_p, C.POINTER(N.SourceDosxyz), C.POINTER(RunParams), C.c_int64,
+                                   C.POINTER(DevOpts), C.POINTER(C.c_double)]
         _i64p = C.POINTER(C.c_int64)
         L.omc_dev_run_beamlets.argtypes = [C.c_void_p, C.POINTER(N.SourceBeamlets), _i64p, C.POINTER(RunParams),
                                            _dp, _dp]
@@ -93,11 +98,12 @@ class Engine:
     """One device context: tables + resident phantoms."""
 
     def __init__(self, precision=FP64, device=0, hbm_budget=0, chunk=0, pipelines=0, yield_trips=0,
-                 drain_lanes=0, drain_trips=0):
+                 drain_lanes=0, drain_trips=0, event_div=0, fetch_min=0):
         L = lib()
         self.ctx = C.c_void_p()
         self.nreg = {}
-        opts = DevOpts(device, precision, hbm_budget, chunk, pipelines, yield_trips, drain_lanes, drain_trips)
+        opts = DevOpts(device, precision, hbm_budget, chunk, pipelines, yield_trips, drain_lanes, drain_trips,
+                       event_div, fetch_min)
         check(L.omc_dev_create(C.byref(self.ctx), C.byref(opts)))
         self.precision = precision
 
@@ -131,6 +137,16 @@ class Engine:
         check(lib().omc_dev_run_dosxyz(self.ctx, C.byref(src), C.byref(rp), s.ctypes.data_as(_dp),
                                        s2.ctypes.data_as(_dp)))
         return s, s2, stats, led
+
+    def tune(self, pid, src, probe, nbatch=10, seed=(97, 33)):
+        """omc_dev_tune: time the dosxyz loop of `src` on `probe` histories
+        under each candidate setting; the engine keeps the fastest.  Returns
+        (settings dict, histories/s)."""
+        rp = RunParams(pid, 1, (C.c_uint32 * 2)(*seed), 0, 0, nbatch, None, None)
+        best = DevOpts()
+        hps = C.c_double(0)
+        check(lib().omc_dev_tune(self.ctx, C.byref(src), C.byref(rp), int(probe), C.byref(best), C.byref(hps)))
+        return best.as_dict(), hps.value
 
     def run_beamlets(self, pid, src, nhist, nbatch, seed=(97, 33), hist0=0):
         """Batch sums of every beamlet: arrays (nbeamlets, nreg)."""
