@@ -33,6 +33,8 @@ extern "C" {
 
 typedef struct omc_dev_ctx omc_dev_ctx;
 
+#define OMC_MAX_NSPLIT 1023   /* photon splitting factor the device supports */
+
 enum { OMC_PREC_FP64 = 0,   /* all transport in FP64 (plain GPU kernel) */
        OMC_PREC_FP32 = 1 }; /* FP32 transport, FP64 tallies and CSDA inversion */
 
@@ -108,7 +110,9 @@ int  omc_dev_release_phantom(omc_dev_ctx *ctx, int phantom_id);
 
 typedef struct {
     int      phantom_id;
-    int      nsplit;            /* photon splitting; the device path requires 1 */
+    int      nsplit;            /* photon splitting (vrt.nsplit, ompmc.c:1903-2118,
+                                   roulette 4149-4164, 4338-4353, 4627-4642):
+                                   1 = off, at most OMC_MAX_NSPLIT */
     uint32_t seed[2];           /* run seed (e.g. the .inp "rng seeds") */
     int64_t  hist0;             /* index of the first history (stream id) */
     int64_t  nperbatch;         /* histories per statistical batch */

@@ -108,10 +108,14 @@ struct Rng {
         ctr++;
         return w >> 8;
     }
-    __device__ __forceinline__ void child(uint32_t &c0, uint32_t &c1) const
+    // Key of a secondary: the Philox block (ctr, tag, 0, 0) under this key.
+    // Interactions use tag 1 for the particles they create; tag 2 is for a
+    // copy of the stream's own particle handed on while the stream goes on
+    // (photon splitting), so the two never share a key.
+    __device__ __forceinline__ void child(uint32_t &c0, uint32_t &c1, uint32_t tag = 1u) const
     {
         uint32_t o[4];
-        philox10(ctr, 1u, 0u, 0u, k0, k1, o);
+        philox10(ctr, tag, 0u, 0u, k0, k1, o);
         c0 = o[0]; c1 = o[1];
     }
 };
@@ -237,6 +241,8 @@ struct DevJob {
     uint64_t hist0;          // stream index of the first history
     int64_t nperbatch;
     uint32_t s0, s1;         // seed
+    int nsplit;              // photon splitting (vrt.nsplit); 0 or 1 = off
+    int pad_nsplit;
     double *tally;           // [slot*nreg + region], slot = batch
     double *ledger;          // optional [3*nhist]
 };

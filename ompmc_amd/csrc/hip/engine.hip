@@ -1107,8 +1107,8 @@ static int check_run(omc_dev_ctx *c, const omc_dev_run_params *rp)
 {
     if (!c || !rp) return fail(OMC_EARG, "null argument");
     OMC_TRY_DEV(check_phantom(c, rp->phantom_id));
-    if (rp->nsplit != 1)
-        return fail(OMC_ENOTSUP, "photon splitting (nsplit=%d) is not supported by the device path", rp->nsplit);
+    if (rp->nsplit < 1 || rp->nsplit > OMC_MAX_NSPLIT)
+        return fail(OMC_EARG, "nsplit %d outside 1..%d", rp->nsplit, OMC_MAX_NSPLIT);
     if (rp->nbatch < 1) return fail(OMC_EARG, "nbatch must be >= 1");
     return OMC_OK;
 }
@@ -1181,6 +1181,7 @@ int omc_dev_run_dosxyz(omc_dev_ctx *c, const omc_source_dosxyz *src, const omc_d
     Work &w = works[0];
     dosxyz_work(w, src);
     stream_fields(w, rp->seed, rp->hist0);
+    w.tmpl.nsplit = rp->nsplit;
     OMC_TRY_DEV(set_histories(w, rp->nperbatch, rp->nbatch));
     w.phantom = rp->phantom_id;
     w.nreg = c->ph[rp->phantom_id].nreg;
@@ -1319,6 +1320,7 @@ static int beamlet_works(omc_dev_ctx *c, const omc_source_beamlets *src, const i
         OMC_TRY_DEV(beamlet_work(w, src, b));
         OMC_TRY_DEV(set_histories(w, nhist[b] / rp->nbatch, rp->nbatch));
         stream_fields(w, rp->seed, h0);
+        w.tmpl.nsplit = rp->nsplit;
         h0 += w.nhist;
         w.index = b;
         w.phantom = rp->phantom_id;

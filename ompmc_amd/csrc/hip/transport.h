@@ -149,6 +149,12 @@ template <typename R> struct PhotonLane {
     // reused while the photon keeps its energy and medium (-2 = none).
     R rhof, gmfpr0;
     int cmed;
+    // Photon splitting in progress (0 = none): nsplit | i_survive << 10 |
+    // (index of the next sub-photon) << 20, the stratification variable
+    // eta' and the mean free paths used so far.  A lane does not yield while
+    // it splits, so none of this is saved.
+    uint32_t split;
+    R eta, dpmfp_old;
 
     // After the particle's voxel or energy was set from outside the step.
     __device__ __forceinline__ void attach(const LaneGeo<R> &G, const Particle<R> &p)
@@ -167,8 +173,96 @@ template <typename R> struct PhotonLane {
         gle = q.st[0][i]; dpmfp = q.st[1][i]; gmfp = q.st[2][i]; cohfac = q.st[3][i]; eig = q.st[4][i];
         int flag, medold;
         unpack_lane(q.sflags[i], phase, flag, imed, medold, lgle);
+        split = 0u;
     }
 };
+
+// A copy of the lane's particle that leaves the lane while the lane's stream
+// goes on (photon splitting) takes a child stream of its own.
+template <typename R>
+__device__ __forceinline__ void detach_stream(const Particle<R> &p, Particle<R> &q)
+{
+    p.rng.child(q.rng.k0, q.rng.k1, 2u);
+    q.rng.ctr = 0u;
+    q.rng.reset_cache();
+}
+
+// Photon splitting (ompmc.c:1903-2118): the photon that enters photon() is
+// followed as nsplit sub-photons of weight wt/nsplit along its direction,
+// each starting where the previous one interacted, over the stratified mean
+// free paths -log(eta') (eta' stepping down by 1/nsplit).  A charged
+// particle a sub-photon sets in motion goes on with weight wt/nsplit; a
+// scattered photon only from the one sub-photon drawn to survive, with its
+// weight restored, and it is split again when it enters photon().  The lane
+// keeps the sub-photon; each interaction works on a copy.
+//
+// Next sub-photon, or false when all are done (ompmc.c:1924-1935).
+template <typename R>
+__device__ __forceinline__ bool next_split(Particle<R> &p, PhotonLane<R> &s, LaneGeo<R> &G)
+{
+    const uint32_t nsplit = s.split & 1023u, isplit = s.split >> 20;
+    if (isplit >= nsplit) return false;
+    s.eta -= (R)1.0 / (R)nsplit;
+    if (s.eta <= (R)0.0) return false;
+    const R dpmfp = -m_log(s.eta) - s.dpmfp_old;
+    s.dpmfp_old += dpmfp;
+    s.dpmfp = dpmfp;
+    s.split += 1u << 20;
+    s.imed = G.medium(p.ir);
+    s.phase = 1;
+    return true;
+}
+
+// Interaction of a sub-photon (ompmc.c:2022-2093), then the next one.
+template <typename R>
+__device__ __forceinline__ bool split_interaction(const KArgs<R> &A, Particle<R> &p, PhotonLane<R> &s,
+                                                  LaneGeo<R> &G)
+{
+    const DevTables<R> &T = A.T;
+    const uint32_t nsplit = s.split & 1023u;
+    const bool survivor = ((s.split >> 10) & 1023u) == (s.split >> 20) - 1u;
+    const int imed = s.imed;
+    Particle<R> q = p;
+    R rnno = urand<R>(q.rng);
+    if (rnno <= (R)1.0 - s.cohfac) {
+        if (survivor) {
+            q.wt *= (R)nsplit;
+            rayleigh<R>(T, q, imed, s.eig, s.gle, s.lgle);
+            p.rng = q.rng;
+            detach_stream<R>(p, q);
+            push<R>(A.qp, q, A.status);
+        } else {
+            p.rng = q.rng;
+        }
+        return next_split<R>(p, s, G);
+    }
+    rnno = urand<R>(q.rng);
+    R4<R> br = T.ph_br[imed * MXGE + s.lgle];
+    R gbr1 = s.gle * br.a1 + br.a0;
+    Particle<R> b;
+    if (rnno <= gbr1 && s.eig > (R)2.0 * (R)RM) {
+        pair<R>(T, q, b, imed);
+        p.rng = q.rng;
+        detach_stream<R>(p, q);
+        push<R>(A.qe, q, A.status);
+        push<R>(A.qe, b, A.status);
+    } else if (rnno < s.gle * br.b1 + br.b0) {
+        compton<R>(q, b);
+        p.rng = q.rng;
+        push<R>(A.qe, b, A.status);
+        if (survivor) {
+            q.wt *= (R)nsplit;
+            detach_stream<R>(p, q);
+            push<R>(A.qp, q, A.status);
+        }
+    } else {
+        photo<R>(q, (R)G.g->ecut[imed + 1]);
+        p.rng = q.rng;
+        detach_stream<R>(p, q);
+        push<R>(A.qe, q, A.status);
+    }
+    return next_split<R>(p, s, G);
+}
 
 // A photon lane alternates transport steps (phase 1) with events: the
 // interaction at the end of a step (phase 2) and the (re-)entry into
@@ -187,6 +281,7 @@ __device__ __forceinline__ bool photon_event(const KArgs<R> &A, Particle<R> &p, 
 {
     const DevTables<R> &T = A.T;
     if (s.phase == 2) {
+        if (s.split) return split_interaction<R>(A, p, s, G);
         const int imed = s.imed;
         R rnno = urand<R>(p.rng);
         if (rnno <= (R)1.0 - s.cohfac) {
@@ -223,6 +318,17 @@ __device__ __forceinline__ bool photon_event(const KArgs<R> &A, Particle<R> &p, 
         return false;
     }
     s.gle = m_log(p.e);
+    const int nsplit = A.jobs[p.job].nsplit;
+    if (nsplit > 1) {
+        R rnno = urand<R>(p.rng);
+        rnno /= (R)nsplit;
+        s.eta = (R)1.0 - rnno + (R)1.0 / (R)nsplit;
+        const uint32_t isurv = (uint32_t)(urand<R>(p.rng) * (R)nsplit);
+        s.split = (uint32_t)nsplit | isurv << 10;
+        s.dpmfp_old = (R)0.0;
+        p.wt /= (R)nsplit;
+        return next_split<R>(p, s, G);
+    }
     R rnno = urand<R>(p.rng);
     rnno /= (R)1.0;
     R d_eta = (R)1.0;
@@ -372,6 +478,38 @@ template <typename R> struct ElectronLane {
 
 enum { E_ALIVE = 0, E_DONE = 1 };
 
+// Photons a charged particle makes go to the photon queue; while photon
+// splitting is on each first plays Russian roulette (ompmc.c:4149-4164,
+// 4338-4353, 4627-4642): it survives with probability 1/nsplit, its weight
+// multiplied by nsplit.  The draws come from the stream of the particle the
+// lane holds.
+template <typename R>
+__device__ __forceinline__ bool roulette(const KArgs<R> &A, Rng &rng, int job, Particle<R> &g)
+{
+    const int ns = A.jobs[job].nsplit;
+    if (ns <= 1) return true;
+    if (urand<R>(rng) * (R)ns > (R)1.0) return false;
+    g.wt *= (R)ns;
+    return true;
+}
+
+template <typename R>
+__device__ __forceinline__ void push_made_photon(const KArgs<R> &A, Particle<R> &p, Particle<R> &b)
+{
+    if (roulette<R>(A, p.rng, p.job, b)) push<R>(A.qp, b, A.status);
+}
+
+// Both annihilation photons: a is the lane's own particle, whose stream makes
+// both roulette draws before it leaves with a.
+template <typename R>
+__device__ __forceinline__ void push_photon_pair(const KArgs<R> &A, Particle<R> &a, Particle<R> &b)
+{
+    const bool ka = roulette<R>(A, a.rng, a.job, a);
+    const bool kb = roulette<R>(A, a.rng, a.job, b);
+    if (ka) push<R>(A.qp, a, A.status);
+    if (kb) push<R>(A.qp, b, A.status);
+}
+
 // electron_cutoff: kinetic energy deposited; a positron then annihilates at
 // rest and the lane hands both photons to the photon queue.
 template <typename R>
@@ -383,8 +521,7 @@ __device__ __forceinline__ void electron_cutoff(const KArgs<R> &A, Particle<R> &
     if (p.iq > 0 && edep < eie) {
         Particle<R> b;
         rannih<R>(p, b);
-        push<R>(A.qp, p, A.status);
-        push<R>(A.qp, b, A.status);
+        push_photon_pair<R>(A, p, b);
     }
 }
 
@@ -421,11 +558,11 @@ __device__ __forceinline__ int electron_interaction(const KArgs<R> &A, Particle<
         R rnno = urand<R>(p.rng);
         if (rnno <= ebr1) {
             brems<R>(T, p, b, m);
-            push<R>(A.qp, b, A.status);
+            push_made_photon<R>(A, p, b);
         } else if (p.e <= (R)T.small->med[m].thmoll) {
             if (ebr1 <= (R)0) return E_ALIVE;   // neither process possible: re-enter
             brems<R>(T, p, b, m);
-            push<R>(A.qp, b, A.status);
+            push_made_photon<R>(A, p, b);
         } else {
             if (moller<R>(T, p, b, m)) push<R>(A.qe, b, A.status);
         }
@@ -435,7 +572,7 @@ __device__ __forceinline__ int electron_interaction(const KArgs<R> &A, Particle<
     R rnno = urand<R>(p.rng);
     if (rnno < pbr1) {
         brems<R>(T, p, b, m);
-        push<R>(A.qp, b, A.status);
+        push_made_photon<R>(A, p, b);
         return E_ALIVE;
     }
     R pbr2 = s.elke * eb.br21 + eb.br20;
@@ -445,8 +582,7 @@ __device__ __forceinline__ int electron_interaction(const KArgs<R> &A, Particle<
         return E_ALIVE;
     }
     annih<R>(p, b);
-    push<R>(A.qp, p, A.status);
-    push<R>(A.qp, b, A.status);
+    push_photon_pair<R>(A, p, b);
     return E_DONE;
 }
 
@@ -679,8 +815,7 @@ __device__ __forceinline__ int electron_iteration(const KArgs<R> &A, Particle<R>
             if (iq > 0 && edep < eie) {
                 Particle<R> b;
                 rannih<R>(p, b);
-                push<R>(A.qp, p, A.status);
-                push<R>(A.qp, b, A.status);
+                push_photon_pair<R>(A, p, b);
             }
             return E_DONE;
         }
@@ -844,6 +979,7 @@ __global__ void __launch_bounds__(256, Waves<R>::photon) k_photon(KArgs<R> A)
                     trips = A.qin.trips[i];
                 } else {
                     s.phase = 0;
+                    s.split = 0u;
                     trips = 0;
                 }
                 G.bind(A, lds_ph, p);
@@ -873,7 +1009,7 @@ __global__ void __launch_bounds__(256, Waves<R>::photon) k_photon(KArgs<R> A)
             if (alive && trips > MAX_ITERS) {           // watchdog: no particle runs forever
                 atomicAdd(&A.status->killed, 1u);
                 alive = false;
-            } else if (alive && (here >= A.yield_trips || (drain && here >= A.drain_trips))) {
+            } else if (alive && s.split == 0u && (here >= A.yield_trips || (drain && here >= A.drain_trips))) {
                 sc.flush(G);
                 yield_push<R>(A.qp, p, s, trips, A.status);
                 nyield++;

@@ -10,7 +10,7 @@
  *
  *   .inp keys: phantom file, pegs file, pgs4form file, data folder,
  *              global ecut, global pcut, spectrum file | mono energy,
- *              ncase (histories per beamlet), nbatch, rng seeds, nsplit (1),
+ *              ncase (histories per beamlet), nbatch, rng seeds, nsplit,
  *              relative dose threshold, beamlet file, output folder
  *   beamlet file: "nBixels nBeams", then nBeams lines "xSource ySource
  *              zSource", then nBixels lines "iBeam xCorner yCorner zCorner
@@ -144,10 +144,8 @@ int main(int argc, char **argv)
     if (omc_inp_get(inp, "global pcut", buf, sizeof buf) != 1) { printf("Can not find 'global pcut' key on input file.\n"); return EXIT_FAILURE; }
     pcut = atof(buf);
     if ((rc = omc_regions_build(&p->reg, &p->ph, &p->tab, ecut, pcut))) drv_die("regions", rc, 0);
-    if (omc_inp_get(inp, "nsplit", buf, sizeof buf) == 1 && atoi(buf) != 1) {
-        fprintf(stderr, "nsplit = %d: photon splitting is not supported by the device path\n", atoi(buf));
-        return EXIT_FAILURE;
-    }
+    int nsplit = 1;
+    if (omc_inp_get(inp, "nsplit", buf, sizeof buf) == 1 && atoi(buf) > 1) nsplit = atoi(buf);
     int seeds[2] = { 97, 33 };
     if (omc_inp_get(inp, "rng seeds", buf, sizeof buf) == 1) sscanf(buf, "%d %d", &seeds[0], &seeds[1]);
     int nhist, nbatch, nperbatch;
@@ -187,7 +185,7 @@ int main(int argc, char **argv)
     omc_dev_run_params rp;
     memset(&rp, 0, sizeof rp);
     rp.phantom_id = pid;
-    rp.nsplit = 1;
+    rp.nsplit = nsplit;
     rp.seed[0] = (uint32_t)seeds[0];
     rp.seed[1] = (uint32_t)seeds[1];
     rp.nbatch = nbatch;

@@ -127,10 +127,10 @@ class Engine:
         self.nreg[pid.value] = regions.nreg
         return pid.value
 
-    def run_dosxyz(self, pid, src, nperbatch, nbatch, seed=(97, 33), hist0=0, nreg=None, ledger=False):
+    def run_dosxyz(self, pid, src, nperbatch, nbatch, seed=(97, 33), hist0=0, nreg=None, ledger=False, nsplit=1):
         stats = DevStats()
         led = np.zeros(3 * nperbatch * nbatch) if ledger else None
-        rp = RunParams(pid, 1, (C.c_uint32 * 2)(*seed), hist0, nperbatch, nbatch,
+        rp = RunParams(pid, nsplit, (C.c_uint32 * 2)(*seed), hist0, nperbatch, nbatch,
                        led.ctypes.data_as(_dp) if ledger else None, C.pointer(stats))
         s = np.zeros(nreg)
         s2 = np.zeros(nreg)
@@ -148,23 +148,23 @@ class Engine:
         check(lib().omc_dev_tune(self.ctx, C.byref(src), C.byref(rp), int(probe), C.byref(best), C.byref(hps)))
         return best.as_dict(), hps.value
 
-    def run_beamlets(self, pid, src, nhist, nbatch, seed=(97, 33), hist0=0):
+    def run_beamlets(self, pid, src, nhist, nbatch, seed=(97, 33), hist0=0, nsplit=1):
         """Batch sums of every beamlet: arrays (nbeamlets, nreg)."""
         nreg = self.nreg[pid]
         nh = np.ascontiguousarray(nhist, dtype=np.int64)
         stats = DevStats()
-        rp = RunParams(pid, 1, (C.c_uint32 * 2)(*seed), hist0, 0, nbatch, None, C.pointer(stats))
+        rp = RunParams(pid, nsplit, (C.c_uint32 * 2)(*seed), hist0, 0, nbatch, None, C.pointer(stats))
         s = np.zeros((src.nbeamlets, nreg))
         s2 = np.zeros((src.nbeamlets, nreg))
         check(lib().omc_dev_run_beamlets(self.ctx, C.byref(src), nh.ctypes.data_as(C.POINTER(C.c_int64)),
                                          C.byref(rp), s.ctypes.data_as(_dp), s2.ctypes.data_as(_dp)))
         return s, s2, stats
 
-    def beamlets_sparse(self, pid, src, nhist, nbatch, rel_thr, seed=(97, 33), hist0=0, cap=None):
+    def beamlets_sparse(self, pid, src, nhist, nbatch, rel_thr, seed=(97, 33), hist0=0, cap=None, nsplit=1):
         """matRad Dij as CSC arrays (jcs, irs, vals) plus the run statistics."""
         nh = np.ascontiguousarray(nhist, dtype=np.int64)
         stats = DevStats()
-        rp = RunParams(pid, 1, (C.c_uint32 * 2)(*seed), hist0, 0, nbatch, None, C.pointer(stats))
+        rp = RunParams(pid, nsplit, (C.c_uint32 * 2)(*seed), hist0, 0, nbatch, None, C.pointer(stats))
         P64 = C.POINTER(C.c_int64)
         nb = src.nbeamlets
         cap = int(cap) if cap is not None else max(1, (self.nreg[pid] - 1) * nb // 50)

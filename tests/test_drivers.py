@@ -40,11 +40,13 @@ def test_usage_error_without_arguments():
         assert r.returncode != 0 and "usage" in r.stderr
 
 
-def test_dosxyz_refuses_photon_splitting(tmp_path):
+@pytest.mark.gpu
+def test_dosxyz_runs_photon_splitting(tmp_path):
     stem = write_case("split5", tmp_path)
     r = run("omc_dosxyz", "-i", stem, "-o", "out")
-    assert r.returncode != 0 and "nsplit = 5" in r.stderr
-    assert not (tmp_path / "out.3ddose").exists()
+    assert r.returncode == 0, r.stderr
+    assert "Photon splitting enabled, nsplit = 5" in r.stdout
+    assert (tmp_path / "out.3ddose").stat().st_size > 0
 
 
 def test_dosxyz_reports_missing_keys(tmp_path):
