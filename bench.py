@@ -249,12 +249,12 @@ def density(prob):
     return np.ctypeslib.as_array(prob.ph.rho, shape=(n,)).copy()
 
 
-def serve_stream(prob, src, beamlets, nh, precision, rate, arrivals, nbatch=10):
+def serve_stream(prob, src, beamlets, nh, precision, rate, arrivals, nbatch=10, engine=None):
     from ompmc_amd import device as D
     nreg = prob.reg.nreg
     rho = density(prob)
-    lat, sig = [], []
-    with D.Engine(precision) as eng:
+    lat, done = [], []
+    with D.Engine(precision, **(engine or {})) as eng:
         eng.upload_media(prob.tab)
         pid = eng.upload_phantom(prob.ph, prob.reg)
         # warm-up request
@@ -277,20 +277,23 @@ def serve_stream(prob, src, beamlets, nh, precision, rate, arrivals, nbatch=10):
                     if res.status != 0:
                         raise RuntimeError(f"request {tk} failed: {res.status}")
                     lat.append(res.finish_ms - res.submit_ms)
-                    sig.append(sigma_at_dmax(s, s2, nbatch, rho))
+                    done.append((s, s2))      # analysed after the stream: the loop only drives the engine
                     total += res.nhist
                     del open_t[tk]
             if not left and not open_t and i < len(arrivals):
                 time.sleep(max(0.0, min(0.002, arrivals[i] - (time.perf_counter() - t0))))
         el = time.perf_counter() - t0
+    sig = [sigma_at_dmax(s, s2, nbatch, rho) for s, s2 in done]
     lat = np.array(lat)
+    if os.environ.get("OMC_BENCH_LATENCIES"):
+        log(f"latencies ms ({'fp32' if precision else 'fp64'}): " + " ".join(f"{v:.0f}" for v in lat))
     return {"requests": len(lat), "histories_per_request": nh, "offered_rate_per_s": rate,
             "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
             "mean_ms": float(lat.mean()), "histories_per_s": total / el,
             "sigma_at_dmax_median": float(np.median(sig)), "sigma_at_dmax_max": float(np.max(sig))}
 
 
-def latency_section(prob, src, nreq=48, load=0.7, seed=77):
+def latency_section(prob, src, nreq=48, load=0.7, seed=77, engine=None):
     """Requests sized for 2% sigma at Dmax; Poisson arrivals at `load` of the
     FP64 engine's closed-loop capacity, the same schedule for both precisions."""
     from ompmc_amd import device as D
@@ -309,7 +312,7 @@ def latency_section(prob, src, nreq=48, load=0.7, seed=77):
     nh -= nh % 10
     beamlets = rng.integers(0, src.nbeamlets, size=nreq)
     # capacity of the FP64 engine on these requests (closed loop)
-    with D.Engine(D.FP64) as eng:
+    with D.Engine(D.FP64, **(engine or {})) as eng:
         eng.upload_media(prob.tab)
         pid = eng.upload_phantom(prob.ph, prob.reg)
         ts = [eng.submit(pid, src, nh, nbatch=10, seed=(50 + k, 1), kind=1, beamlet=int(beamlets[k]))
@@ -326,7 +329,7 @@ def latency_section(prob, src, nreq=48, load=0.7, seed=77):
            "load_of_fp64_capacity": load}
     for name, prec in (("fp32", D.FP32), ("fp64", D.FP64)):
         log(f"latency stream {name}: {nreq} requests of {nh} histories at {rate:.2f}/s")
-        out[name] = serve_stream(prob, src, beamlets, nh, prec, rate, arrivals)
+        out[name] = serve_stream(prob, src, beamlets, nh, prec, rate, arrivals, engine=engine)
     return out
 
 

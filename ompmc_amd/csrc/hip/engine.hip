@@ -161,6 +161,9 @@ struct RunSink {
     int rc = OMC_OK;
 };
 
+// Stages of collecting a finished job's results on the helper stream.
+enum { FIN_NONE = 0, FIN_QUEUED, FIN_FOLD, FIN_COLUMN };
+
 struct JobRec {
     bool used = false;
     Work *w = nullptr;
@@ -172,6 +175,8 @@ struct JobRec {
     double *cdf = nullptr;
     double *ledger = nullptr;
     size_t ledger_bytes = 0;
+    int fin = FIN_NONE;           // result collection, see pump_finish
+    uint32_t nnz = 0;             // sparse column length, once known
 };
 
 }  // namespace
@@ -203,6 +208,13 @@ struct omc_dev_ctx {
     double *fin_sums = nullptr;       // [2*nreg]
     SparseScratch sp = {};
     size_t fin_bytes = 0;
+    // pinned host side of the finishing copies, same sizes
+    double *h_sums = nullptr;         // [2*nreg]
+    int64_t *h_rows = nullptr;        // [nvox]
+    double *h_vals = nullptr;         // [nvox]
+    uint32_t *h_nnz = nullptr;
+    std::deque<int> finishing;        // jobs whose results are being collected, oldest first
+    hipEvent_t ev_fin = nullptr;      // end of the head job's current finishing stage
     // jobs
     DevJob *d_jobs = nullptr;
     JobRec jobs[MAX_JOBS];
@@ -431,11 +443,16 @@ template <typename R> static DevTables<R> tables_view(omc_dev_ctx *c)
 // --------------------------------------------------------------- phantoms --
 // Scratch for finishing a job on the device: the folded sums and the sparse
 // column of a beamlet, sized for the largest resident phantom.
+static int drain_finish(omc_dev_ctx *c);
+
 static int grow_finish_scratch(omc_dev_ctx *c, int64_t nvox)
 {
     if (nvox <= c->fin_nvox) return OMC_OK;
+    OMC_TRY_DEV(drain_finish(c));
     if (c->fin_sums) dfree(c, c->fin_sums, c->fin_bytes);
+    if (c->h_sums) (void)hipHostFree(c->h_sums);
     c->fin_sums = nullptr;
+    c->h_sums = nullptr;
     c->fin_nvox = -1;
     const int nblk = sparse_blocks(nvox);
     const size_t sums = 2 * sizeof(double) * (size_t)(nvox + 1);
@@ -452,6 +469,13 @@ static int grow_finish_scratch(omc_dev_ctx *c, int64_t nvox)
     c->sp.blk = (uint32_t *)p;              p += up(blk);
     c->sp.rows = (int64_t *)p;              p += up(rows);
     c->sp.vals = (double *)p;
+    char *h = nullptr;
+    if (hipHostMalloc((void **)&h, up(sums) + up(rows) + up(vals) + up(4), hipHostMallocDefault) != hipSuccess)
+        return fail(OMC_EDEVICE, "pinned finishing buffers (%zu bytes)", up(sums) + up(rows) + up(vals) + up(4));
+    c->h_sums = (double *)h;                h += up(sums);
+    c->h_rows = (int64_t *)h;               h += up(rows);
+    c->h_vals = (double *)h;                h += up(vals);
+    c->h_nnz = (uint32_t *)h;
     c->fin_nvox = nvox;
     return OMC_OK;
 }
@@ -614,59 +638,138 @@ static int admit(omc_dev_ctx *c)
     return OMC_OK;
 }
 
-// The job's last particle is gone: fold its batches (accumEndep,
-// omc_dosxyz.c:696-717) and deliver the result.
-static int finish_job(omc_dev_ctx *c, int j)
+// Finishing a job: its last particle is gone, so its tally is final.  The
+// results are collected on the helper stream while the pipelines go on:
+//   FIN_FOLD    fold the batches (accumEndep, omc_dosxyz.c:696-717) and/or
+//               build the beamlet's sparse column; copy the sums and the
+//               column length to pinned memory
+//   FIN_COLUMN  copy the column's rows and values to pinned memory
+// then the host delivers the result.  The device scratch is shared, so one
+// job is collected at a time, in the order the jobs ended.
+static bool wants_sums(const Work *w)
+{
+    const RunSink *k = w->sink;
+    return !k || k->sum || !k->beamlets;
+}
+
+static int finish_fold(omc_dev_ctx *c, int j)
 {
     JobRec &J = c->jobs[j];
-    Work *w = J.w;
+    const Work *w = J.w;
+    const RunSink *k = w->sink;
     const int64_t nreg = w->nreg;
-    const PhantomSlot &P = c->ph[w->phantom];
     hipStream_t s = c->aux;
-    int rc = OMC_OK;
-    RunSink *k = w->sink;
-    const bool want_sums = !k || k->sum || !k->beamlets;
-    if (want_sums) {
+    if (wants_sums(w)) {
         launch_fold(J.tally, w->nbatch, nreg, c->fin_sums, c->fin_sums + nreg, s);
-        std::vector<double> h(2 * (size_t)nreg);
-        if (hipMemcpyAsync(h.data(), c->fin_sums, h.size() * sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess ||
-            hipStreamSynchronize(s) != hipSuccess)
-            rc = fail(OMC_EDEVICE, "copy tallies");
-        if (!rc && !k) {
-            w->sum.assign(h.begin(), h.begin() + nreg);
-            w->sum2.assign(h.begin() + nreg, h.end());
-        } else if (!rc && k->sum) {
-            double *s1 = k->sum + (k->beamlets ? (size_t)w->index * nreg : 0);
-            double *s2 = k->sum2 + (k->beamlets ? (size_t)w->index * nreg : 0);
-            for (int64_t i = 0; i < nreg; ++i) { s1[i] += h[i]; s2[i] += h[nreg + i]; }
-        }
+        if (hipMemcpyAsync(c->h_sums, c->fin_sums, 2 * (size_t)nreg * sizeof(double), hipMemcpyDeviceToHost, s) !=
+            hipSuccess)
+            return fail(OMC_EDEVICE, "copy tallies");
     }
-    if (!rc && k && k->ledger && J.ledger &&
-        hipMemcpy(k->ledger, J.ledger, J.ledger_bytes, hipMemcpyDeviceToHost) != hipSuccess)
-        rc = fail(OMC_EDEVICE, "copy ledger");
-    if (!rc && k && k->sparse) {
+    if (k && k->sparse) {
+        const PhantomSlot &P = c->ph[w->phantom];
         const double inc = (double)(w->nperbatch * (int64_t)w->nbatch);
         launch_beamlet_column(J.tally, w->nbatch, P.isize, P.jsize, P.ksize, (const double *)P.xb.p,
                               (const double *)P.yb.p, (const double *)P.zb.p, (const double *)P.rho.p, inc,
                               k->rel_thr, c->sp, s);
         const int nblk = sparse_blocks((int64_t)P.isize * P.jsize * P.ksize);
-        uint32_t nnz = 0;
-        if (hipMemcpyAsync(&nnz, c->sp.blk + nblk, sizeof nnz, hipMemcpyDeviceToHost, s) != hipSuccess ||
-            hipStreamSynchronize(s) != hipSuccess)
-            rc = fail(OMC_EDEVICE, "sparse count");
-        if (!rc) {
-            auto &rows = k->rows[w->index];
-            auto &vals = k->vals[w->index];
-            rows.resize(nnz);
-            vals.resize(nnz);
-            if (nnz && (hipMemcpy(rows.data(), c->sp.rows, nnz * sizeof(int64_t), hipMemcpyDeviceToHost) != hipSuccess ||
-                        hipMemcpy(vals.data(), c->sp.vals, nnz * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess))
-                rc = fail(OMC_EDEVICE, "copy sparse column");
+        if (hipMemcpyAsync(c->h_nnz, c->sp.blk + nblk, sizeof(uint32_t), hipMemcpyDeviceToHost, s) != hipSuccess)
+            return fail(OMC_EDEVICE, "sparse count");
+    }
+    if (hipEventRecord(c->ev_fin, s) != hipSuccess) return fail(OMC_EDEVICE, "finishing event");
+    J.fin = FIN_FOLD;
+    return OMC_OK;
+}
+
+static int finish_column(omc_dev_ctx *c, int j)
+{
+    JobRec &J = c->jobs[j];
+    hipStream_t s = c->aux;
+    J.nnz = *c->h_nnz;
+    if (J.nnz > (uint32_t)c->fin_nvox) return fail(OMC_EDEVICE, "sparse column of %u entries", J.nnz);
+    if (J.nnz && (hipMemcpyAsync(c->h_rows, c->sp.rows, J.nnz * sizeof(int64_t), hipMemcpyDeviceToHost, s) !=
+                      hipSuccess ||
+                  hipMemcpyAsync(c->h_vals, c->sp.vals, J.nnz * sizeof(double), hipMemcpyDeviceToHost, s) != hipSuccess))
+        return fail(OMC_EDEVICE, "copy sparse column");
+    if (hipEventRecord(c->ev_fin, s) != hipSuccess) return fail(OMC_EDEVICE, "finishing event");
+    J.fin = FIN_COLUMN;
+    return OMC_OK;
+}
+
+// Host side: hand the collected results to the work's owner.
+static int finish_deliver(omc_dev_ctx *c, int j)
+{
+    JobRec &J = c->jobs[j];
+    Work *w = J.w;
+    RunSink *k = w->sink;
+    const int64_t nreg = w->nreg;
+    if (wants_sums(w)) {
+        const double *h = c->h_sums;
+        if (!k) {
+            w->sum.assign(h, h + nreg);
+            w->sum2.assign(h + nreg, h + 2 * nreg);
+        } else if (k->sum) {
+            double *s1 = k->sum + (k->beamlets ? (size_t)w->index * nreg : 0);
+            double *s2 = k->sum2 + (k->beamlets ? (size_t)w->index * nreg : 0);
+            for (int64_t i = 0; i < nreg; ++i) { s1[i] += h[i]; s2[i] += h[nreg + i]; }
         }
     }
-    release_job(c, j);
-    end_work(c, w, rc);
-    return rc;
+    if (k && k->ledger && J.ledger &&
+        hipMemcpy(k->ledger, J.ledger, J.ledger_bytes, hipMemcpyDeviceToHost) != hipSuccess)
+        return fail(OMC_EDEVICE, "copy ledger");
+    if (k && k->sparse) {
+        k->rows[w->index].assign(c->h_rows, c->h_rows + J.nnz);
+        k->vals[w->index].assign(c->h_vals, c->h_vals + J.nnz);
+    }
+    return OMC_OK;
+}
+
+// Advances the collection of finished jobs without blocking.  Returns the
+// number of jobs delivered, or an error (the failing job is ended with it).
+static int pump_finish(omc_dev_ctx *c)
+{
+    int delivered = 0;
+    while (!c->finishing.empty()) {
+        const int j = c->finishing.front();
+        JobRec &J = c->jobs[j];
+        int rc = OMC_OK;
+        if (J.fin == FIN_QUEUED) {
+            rc = finish_fold(c, j);
+        } else {
+            const hipError_t e = hipEventQuery(c->ev_fin);
+            if (e == hipErrorNotReady) return delivered;
+            if (e != hipSuccess) rc = fail(OMC_EDEVICE, "finishing: %s", hipGetErrorString(e));
+            else if (J.fin == FIN_FOLD && J.w->sink && J.w->sink->sparse) rc = finish_column(c, j);
+            else {
+                rc = finish_deliver(c, j);
+                if (!rc) {
+                    Work *w = J.w;
+                    c->finishing.pop_front();
+                    release_job(c, j);
+                    end_work(c, w, OMC_OK);
+                    delivered++;
+                }
+            }
+        }
+        if (rc) {
+            Work *w = J.w;
+            c->finishing.pop_front();
+            release_job(c, j);
+            end_work(c, w, rc);
+            return rc;
+        }
+    }
+    return delivered;
+}
+
+// Collects every finished job, waiting for the helper stream.
+static int drain_finish(omc_dev_ctx *c)
+{
+    while (!c->finishing.empty()) {
+        HIPCHK(hipStreamSynchronize(c->aux));
+        const int r = pump_finish(c);
+        if (r < 0) return r;
+    }
+    return OMC_OK;
 }
 
 // Hand up to `room` new primaries to pipeline p (round robin over the
@@ -814,15 +917,14 @@ static int complete_generation(omc_dev_ctx *c, int pi)
         check.push_back(j);
     }
     P.open.swap(still);
-    int rc = OMC_OK;
     for (int j : check) {
         JobRec &J = c->jobs[j];
-        if (J.used && J.pipes == 0 && J.injected == J.w->nhist) {
-            int r = finish_job(c, j);
-            if (r && !rc) rc = r;
+        if (J.used && J.fin == FIN_NONE && J.pipes == 0 && J.injected == J.w->nhist) {
+            J.fin = FIN_QUEUED;
+            c->finishing.push_back(j);
         }
     }
-    return rc;
+    return OMC_OK;
 }
 
 // Stops everything after a fatal error: waits for the pipelines, empties
@@ -831,6 +933,8 @@ static void reset_engine(omc_dev_ctx *c, int rc)
 {
     char keep[sizeof g_err];
     memcpy(keep, g_err, sizeof keep);
+    (void)hipStreamSynchronize(c->aux);
+    c->finishing.clear();
     for (auto &P : c->pipes) {
         (void)hipStreamSynchronize(P.stream);
         (void)hipMemsetAsync(P.d_counts, 0, 8 * sizeof(uint32_t), P.stream);
@@ -859,8 +963,9 @@ static void reset_engine(omc_dev_ctx *c, int rc)
 }
 
 // One scheduler turn: admit, start a generation on every idle pipeline that
-// has work, wait for one generation to end and process it.  Returns 1 if a
-// generation was processed, 0 if the engine is idle.
+// has work, then wait until a generation ends (and process it) or a finished
+// job's results are delivered.  Returns 1 after either, 0 if the engine is
+// idle.
 template <typename R> static int turn(omc_dev_ctx *c)
 {
     auto h0 = std::chrono::steady_clock::now();
@@ -869,11 +974,18 @@ template <typename R> static int turn(omc_dev_ctx *c)
         if (!c->pipes[p].inflight) rc = launch_generation<R>(c, (int)p);
     c->acc.host_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - h0).count();
     if (rc) { reset_engine(c, rc); return rc; }
-    bool any = false;
+    bool any = !c->finishing.empty();
     for (auto &P : c->pipes) any |= P.inflight;
     if (!any) return 0;
     int spins = 0;
     for (;;) {
+        if (!c->finishing.empty()) {
+            auto h1 = std::chrono::steady_clock::now();
+            const int d = pump_finish(c);
+            c->acc.host_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - h1).count();
+            if (d < 0) { reset_engine(c, d); return d; }
+            if (d > 0) return 1;
+        }
         int oldest = -1;
         for (size_t p = 0; p < c->pipes.size(); ++p) {
             Pipe &P = c->pipes[p];
@@ -1034,6 +1146,7 @@ int omc_dev_create(omc_dev_ctx **out, const omc_dev_opts *opts)
     c->budget = o.hbm_budget > 0 ? o.hbm_budget : (int64_t)(0.9 * (double)freeb);
     int rc = OMC_OK;
     if (hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking) != hipSuccess) rc = fail(OMC_EDEVICE, "stream");
+    if (!rc && hipEventCreateWithFlags(&c->ev_fin, hipEventDisableTiming) != hipSuccess) rc = fail(OMC_EDEVICE, "event");
     if (!rc) rc = dalloc(c, (void **)&c->d_jobs, MAX_JOBS * sizeof(DevJob));
     const size_t phb = o.precision == OMC_PREC_FP64 ? sizeof(DevPhantom<double>) : sizeof(DevPhantom<float>);
     if (!rc) rc = dalloc(c, &c->d_phantoms, MAX_PHANTOMS * phb);
@@ -1049,7 +1162,10 @@ int omc_dev_destroy(omc_dev_ctx *c)
     (void)hipSetDevice(c->device);
     for (auto &P : c->pipes) free_pipe(P);
     if (c->aux) (void)hipStreamSynchronize(c->aux);
+    c->finishing.clear();
     for (int j = 0; j < c->jobs_hi; ++j) if (c->jobs[j].used) release_job(c, j);
+    if (c->h_sums) (void)hipHostFree(c->h_sums);
+    if (c->ev_fin) (void)hipEventDestroy(c->ev_fin);
     for (auto &kv : c->pool) (void)hipFree(kv.second);
     void *ptrs[] = { c->small, c->ph_mfp, c->ph_br, c->ph_pmax, c->el, c->csda, c->spin, c->ms,
                      c->d_jobs, c->d_phantoms, c->fin_sums };

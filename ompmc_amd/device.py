@@ -200,7 +200,7 @@ class Engine:
         if nreg is None:
             check(lib().omc_dev_poll(self.ctx, ticket, C.byref(res), None, None))
             return res, None, None
-        s = np.zeros(nreg)
-        s2 = np.zeros(nreg)
+        s = np.empty(nreg)        # written in full once the ticket is done
+        s2 = np.empty(nreg)
         check(lib().omc_dev_poll(self.ctx, ticket, C.byref(res), s.ctypes.data_as(_dp), s2.ctypes.data_as(_dp)))
         return (res, s, s2) if res.done and res.status == 0 else (res, None, None)

@@ -4,6 +4,8 @@ Requests share the device's transport generations; every history keeps its
 own Philox stream, so a request's result must not depend on what else is in
 flight: alone and in a mix it gives the same batch sums up to FP64
 summation order.  Admission follows the HBM budget."""
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -109,3 +111,60 @@ def test_request_errors(D, problems):
         with pytest.raises(RuntimeError, match="ticket"):
             eng.poll(12345)
         assert eng.step() == 0                  # idle engine
+
+
+def _first_slices(ph, reg, k):
+    """The phantom's first k slices along z, sharing its arrays (voxel index
+    ix + iy*isize + iz*isize*jsize, so they are a prefix)."""
+    sub = N.Phantom()
+    C.pointer(sub)[0] = ph
+    sub.ksize = k
+    r = N.Regions()
+    C.pointer(r)[0] = reg
+    r.nreg = ph.isize * ph.jsize * k + 1
+    return sub, r
+
+
+def test_larger_phantom_uploaded_while_requests_finish(D, problems):
+    """Uploading a phantom larger than any resident one regrows the scratch
+    that finished requests are collected through; requests already finishing
+    must be delivered intact first, and the new phantom must serve at once."""
+    pp, _ = problems
+    small, sreg = _first_slices(pp.ph, pp.reg, 3)
+    nreg_s = sreg.nreg
+    seeds = [(11, k) for k in range(6)]
+    with D.Engine(0) as eng:                        # each request alone
+        eng.upload_media(pp.tab)
+        sid = eng.upload_phantom(small, sreg)
+        ref = {}
+        for sd in seeds:
+            t = eng.submit(sid, pp.src, 4000, nbatch=4, seed=sd)
+            ref[sd] = drain(eng, [t], {t: nreg_s})[t]
+    with D.Engine(0) as eng:
+        eng.upload_media(pp.tab)
+        sid = eng.upload_phantom(small, sreg)
+        tick = {eng.submit(sid, pp.src, 4000, nbatch=4, seed=sd): sd for sd in seeds}
+        got = {}
+        for _ in range(200000):                     # until the first request is delivered
+            eng.step()
+            for t in list(tick):
+                res, s, s2 = eng.poll(t, nreg_s)
+                if res.done:
+                    got[tick.pop(t)] = (res, s, s2)
+            if got:
+                break
+        assert got and tick, "the upload must happen with requests still on the device"
+        pid = eng.upload_phantom(pp.ph, pp.reg)
+        tp = eng.submit(pid, pp.src, 6000, nbatch=4, seed=(5, 5))
+        rest = drain(eng, list(tick) + [tp], {**{t: nreg_s for t in tick}, tp: pp.reg.nreg})
+        got.update({tick[t]: r for t, r in rest.items() if t != tp})
+    assert sorted(got) == sorted(seeds)
+    for sd in seeds:
+        r, s, s2 = got[sd]
+        assert r.status == 0
+        scale = np.abs(ref[sd][1]).max()
+        assert scale > 0
+        assert np.allclose(s, ref[sd][1], rtol=1e-12, atol=1e-12 * scale)
+        assert np.allclose(s2, ref[sd][2], rtol=1e-12, atol=1e-12 * np.abs(ref[sd][2]).max())
+    rp, sp_, _ = rest[tp]
+    assert rp.status == 0 and sp_[1:].sum() > 0
