@@ -125,7 +125,7 @@ def autotune_section(prob, precision, probe=4_000_000):
         t0 = time.perf_counter()
         best, hps = eng.tune(pid, prob.src, probe, nbatch=10, seed=(4, 4))
         el = time.perf_counter() - t0
-    keys = ("pipelines", "chunk", "yield_trips", "drain_lanes", "drain_trips", "event_div", "fetch_min")
+    keys = ("pipelines", "chunk", "yield_trips", "drain_lanes", "drain_trips", "drain_min", "event_div", "fetch_min")
     return {"probe_histories": probe, "seconds": el, "histories_per_s": hps,
             "chosen": {k: best[k] for k in keys}}
 
@@ -167,7 +167,7 @@ def roofline(stats, precision):
         "frac": achieved / HBM_PEAK_GBS, "traffic": ratio * alg_launch if ratio else None,
         "traffic_over_algorithmic": ratio,
         "traffic_source": ({"file": f"profiles/{name}_slab6mv_fp32_traffic.md",
-                            "profiled_run": "1e6 histories, same slab workload",
+                            "profiled_run": "this slab workload at the same size (2e7 histories per step)",
                             "measured_bytes_per_launch": measured,
                             "algorithmic_bytes_per_launch": alg_profiled,
                             "here": "measured/algorithmic of the profiled run x this run's algorithmic bytes per launch"}

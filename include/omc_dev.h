@@ -62,6 +62,12 @@ typedef struct {
     int     fetch_min;       /* free lanes of a wavefront take new particles
                                 together once this many are free;
                                 0 = automatic */
+    int64_t drain_min;       /* launches whose input holds fewer particles
+                                than this do not drain: on a device they
+                                cannot fill, repacking stragglers gains
+                                nothing and costs a launch per repack;
+                                0 = automatic (the lanes the device holds
+                                resident), < 0 = every launch drains */
 } omc_dev_opts;
 
 /* Device counters of one run (all sums over the run). */

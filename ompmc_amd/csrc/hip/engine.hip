@@ -80,6 +80,15 @@ constexpr uint32_t DEFAULT_YIELD_TRIPS = 4096;
 // 38 -> 55M histories/s, FP64 37 -> 42M).
 constexpr uint32_t DEFAULT_DRAIN_LANES = 32;
 constexpr uint32_t DEFAULT_DRAIN_TRIPS = 16;
+// Repacking only pays while the device is full: a launch whose input is
+// smaller than the lanes the device holds resident (CUs x 4 SIMDs x the
+// electron kernel's waves per SIMD x 64) runs without draining.  A lone
+// 2e4-history job otherwise takes ~40 short generations, its stragglers
+// handed on every 16 trips.
+constexpr int64_t drain_min_default(int cus, int precision)
+{
+    return (int64_t)cus * 4 * (precision == OMC_PREC_FP64 ? 2 : 3) * 64;
+}
 // A photon wavefront runs its waiting events (interaction, re-entry) once
 // half its live lanes wait for one (slab sweep of 2/4/8/16: 2 first, 22%
 // more histories/s than running each event as it comes); free lanes take new
@@ -191,6 +200,7 @@ struct omc_dev_ctx {
     uint32_t qcap = 0;
     uint32_t yield_trips = 0;
     uint32_t drain_lanes = 0, drain_trips = 0;
+    int64_t drain_min = 0;            // smallest launch input that drains
     uint32_t event_div = 0, fetch_min = 0;
     int cus = 256;
     std::chrono::steady_clock::time_point t0;
@@ -836,7 +846,7 @@ template <typename R> static int launch_generation(omc_dev_ctx *c, int pi)
     A.qe = queue_view<R>(c, P, 2 + nxt);
     A.status = P.d_status;
     A.yield_trips = c->yield_trips;
-    A.drain_lanes = c->drain_lanes;
+    A.drain_lanes = c->drain_lanes;     // set per launch below
     A.drain_trips = c->drain_trips;
     A.event_div = c->event_div;
     A.fetch_min = c->fetch_min;
@@ -844,6 +854,7 @@ template <typename R> static int launch_generation(omc_dev_ctx *c, int pi)
     P.had_ph = inP > 0;
     if (P.had_ph) {
         A.qin = qpc;
+        A.drain_lanes = (int64_t)inP >= c->drain_min ? c->drain_lanes : 0u;
         A.in_count = P.d_counts + P.cur;
         A.fetch = P.d_counts + 4;
         HIPCHK(hipEventRecord(P.ev_ph0, s));
@@ -853,6 +864,7 @@ template <typename R> static int launch_generation(omc_dev_ctx *c, int pi)
     P.had_el = inE > 0;
     if (P.had_el) {
         A.qin = qec;
+        A.drain_lanes = (int64_t)inE >= c->drain_min ? c->drain_lanes : 0u;
         A.in_count = P.d_counts + 2 + P.cur;
         A.fetch = P.d_counts + 5;
         HIPCHK(hipEventRecord(P.ev_el0, s));
@@ -1141,6 +1153,7 @@ int omc_dev_create(omc_dev_ctx **out, const omc_dev_opts *opts)
     c->fetch_min = o.fetch_min > 0 ? (uint32_t)std::min(o.fetch_min, 64) : DEFAULT_FETCH_MIN;
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, o.device) == hipSuccess) c->cus = prop.multiProcessorCount;
+    c->drain_min = o.drain_min > 0 ? o.drain_min : o.drain_min < 0 ? 0 : drain_min_default(c->cus, c->precision);
     size_t freeb = 0, totalb = 0;
     (void)hipMemGetInfo(&freeb, &totalb);
     c->budget = o.hbm_budget > 0 ? o.hbm_budget : (int64_t)(0.9 * (double)freeb);
@@ -1413,6 +1426,7 @@ int omc_dev_tune(omc_dev_ctx *c, const omc_source_dosxyz *src, const omc_dev_run
     best->yield_trips = (int)c->yield_trips;
     best->drain_lanes = (int)c->drain_lanes;
     best->drain_trips = (int)c->drain_trips;
+    best->drain_min = c->drain_min > 0 ? c->drain_min : -1;
     best->event_div = (int)c->event_div;
     best->fetch_min = (int)c->fetch_min;
     *hps = bv;

@@ -16,7 +16,8 @@ _dp = C.POINTER(C.c_double)
 class DevOpts(C.Structure):
     _fields_ = [("device", C.c_int), ("precision", C.c_int), ("hbm_budget", C.c_int64), ("chunk", C.c_int64),
                 ("pipelines", C.c_int), ("yield_trips", C.c_int), ("drain_lanes", C.c_int),
-                ("drain_trips", C.c_int), ("event_div", C.c_int), ("fetch_min", C.c_int)]
+                ("drain_trips", C.c_int), ("event_div", C.c_int), ("fetch_min", C.c_int),
+                ("drain_min", C.c_int64)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -98,12 +99,12 @@ class Engine:
     """One device context: tables + resident phantoms."""
 
     def __init__(self, precision=FP64, device=0, hbm_budget=0, chunk=0, pipelines=0, yield_trips=0,
-                 drain_lanes=0, drain_trips=0, event_div=0, fetch_min=0):
+                 drain_lanes=0, drain_trips=0, event_div=0, fetch_min=0, drain_min=0):
         L = lib()
         self.ctx = C.c_void_p()
         self.nreg = {}
         opts = DevOpts(device, precision, hbm_budget, chunk, pipelines, yield_trips, drain_lanes, drain_trips,
-                       event_div, fetch_min)
+                       event_div, fetch_min, drain_min)
         check(L.omc_dev_create(C.byref(self.ctx), C.byref(opts)))
         self.precision = precision
 

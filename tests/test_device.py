@@ -193,16 +193,21 @@ def test_schedule_and_reruns_do_not_change_histories(oracle, D, tmp_path, case, 
     pipelines, a small refill chunk (many injections, primaries mixed with
     older secondaries), particles handed to the next launch after every
     one or three loop trips (their lane state travels with them), handed
-    on after every trip once a launch's input is used up, or a repeat
-    reproduce the same energies up to FP64 summation order."""
+    on after every trip once a launch's input is used up, not drained at
+    all, or a repeat reproduce the same energies up to FP64 summation
+    order."""
     stem = write_case(case, tmp_path, ncase=3000)
     prob = oracle.load_problem(stem)
     try:
         out = []
         for chunk, pipes, ytrips, drain in ((0, 0, 0, -1), (0, 1, 0, 0), (777, 3, 0, 0), (0, 2, 1, 0),
-                                            (5000, 0, 3, 0), (2000, 2, 0, 64)):
-            with D.Engine(precision, chunk=chunk, pipelines=pipes, yield_trips=ytrips, drain_lanes=drain,
-                          drain_trips=1 if drain == 64 else 0) as eng:
+                                            (5000, 0, 3, 0), (2000, 2, 0, 64), (0, 0, 0, None)):
+            # drain_min -1: these small launches drain too (by default, the
+            # last configuration, a launch smaller than the device's
+            # resident lanes does not)
+            dm = 0 if drain is None or drain < 0 else -1
+            with D.Engine(precision, chunk=chunk, pipelines=pipes, yield_trips=ytrips, drain_lanes=drain or 0,
+                          drain_trips=1 if drain == 64 else 0, drain_min=dm) as eng:
                 eng.upload_media(prob.tab)
                 pid = eng.upload_phantom(prob.ph, prob.reg)
                 s, _, st, led = eng.run_dosxyz(pid, prob.src, prob.nperbatch, prob.nbatch, seed=(4, 4),
@@ -212,6 +217,7 @@ def test_schedule_and_reruns_do_not_change_histories(oracle, D, tmp_path, case, 
         oracle.free_problem(prob)
     assert out[2][2] >= 4                       # 3000 histories, chunk 777
     assert out[3][3] > 1000 and out[4][3] > 100 and out[5][3] > 100
+    assert out[6][3] <= 5, "a job this small must not drain by default"
     for s, led, _, _ in out[1:]:
         assert np.allclose(s, out[0][0], rtol=1e-12, atol=1e-12 * out[0][0].max())
         assert np.allclose(led, out[0][1], rtol=1e-12, atol=1e-12)
