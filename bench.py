@@ -90,7 +90,7 @@ def patient_problem(workdir: Path):
 
 
 # -------------------------------------------------------------- slab run --
-def time_slab(prob, precision, hist, steps, warmup, pipelines=0, chunk=0):
+def time_slab(prob, precision, hist, steps, warmup, pipelines=0, chunk=0, seed0=2000):
     from ompmc_amd import device as D
     nreg = prob.reg.nreg
     nbatch = 10
@@ -104,14 +104,50 @@ def time_slab(prob, precision, hist, steps, warmup, pipelines=0, chunk=0):
         while w < warmup or time.perf_counter() - t_w < 3.0:
             eng.run_dosxyz(pid, prob.src, nper, nbatch, seed=(1000 + w, 7), nreg=nreg)
             w += 1
-        stats, etot = [], []
+        stats, etot, sums = [], [], []
         t0 = time.perf_counter()
         for k in range(steps):
-            s, _, st, _ = eng.run_dosxyz(pid, prob.src, nper, nbatch, seed=(2000 + k, 7), nreg=nreg)
+            s, s2, st, _ = eng.run_dosxyz(pid, prob.src, nper, nbatch, seed=(seed0 + k, 7), nreg=nreg)
             stats.append(st.as_dict())
-            etot.append(float(s[1:].sum()))
+            sums.append((s, s2))
         el = time.perf_counter() - t0
-    return el, stats, etot, nper * nbatch
+    etot = [float(s[1:].sum()) for s, _ in sums]
+    # the steps' batches together: steps x nbatch batches of nper histories
+    batches = (sum(s for s, _ in sums), sum(s2 for _, s2 in sums), steps * nbatch)
+    return el, stats, etot, nper * nbatch, batches
+
+
+def accuracy_gates(prob, a, b):
+    """FP32 run `a` against FP64 run `b` (independent seeds), each as
+    (sum, sum of squares, batches) per region: the fraction of voxels above
+    10% of the FP64 Dmax whose batch means agree within two combined
+    standard errors, and the ratio of the integral energies."""
+    rho = density(prob)
+
+    def mean_var(s, s2, nb):
+        m = s[1:] / nb
+        return m, np.maximum(s2[1:] / nb - m * m, 0.0) / (nb - 1)
+
+    from scipy import stats
+    ma, va = mean_var(*a)
+    mb, vb = mean_var(*b)
+    dose_b = np.where(rho < 0.044, 0.0, mb / np.maximum(rho, 1e-30))
+    sel = dose_b > 0.1 * dose_b.max()
+    n = int(sel.sum())
+    frac = float(np.mean(np.abs(ma - mb)[sel] <= 2.0 * np.sqrt(va + vb)[sel]))
+    ratio = float(a[0][1:].sum() / b[0][1:].sum())
+    # sigma is estimated from the batches, so even two identical engines put
+    # only P(|t| <= 2) of the voxels within 2 sigma (94.98% at 58 degrees of
+    # freedom); below 95% the gate allows three binomial standard deviations
+    # under that expectation
+    expect = float(2.0 * stats.t.cdf(2.0, a[2] + b[2] - 2) - 1.0)
+    floor = min(0.95, expect - 3.0 * math.sqrt(expect * (1.0 - expect) / max(n, 1)))
+    return {"voxels_above_10pct_dmax": n, "fraction_within_2sigma": frac,
+            "expected_if_identical": expect, "integral_energy_fp32_over_fp64": ratio,
+            "batches": [a[2], b[2]],
+            "gates": "at least 95% of those voxels within 2 sigma (or within 3 binomial sd of the coverage "
+                     "expected with batch-estimated sigma, if that is below 95%), integral energy within 0.5%",
+            "pass": bool(frac >= floor and abs(ratio - 1.0) <= 0.005)}
 
 
 def autotune_section(prob, precision, probe=4_000_000):
@@ -397,7 +433,8 @@ def main():
     work = Path(tempfile.mkdtemp(prefix="omc_bench_"))
     prob, stem = slab_problem(work, args.hist)
     log(f"slab FP32: {args.steps} steps of {args.hist} histories")
-    el32, st32, e32, nh = time_slab(prob, D.FP32, args.hist, args.steps, args.warmup, args.pipelines, args.chunk)
+    el32, st32, e32, nh, b32 = time_slab(prob, D.FP32, args.hist, args.steps, args.warmup, args.pipelines,
+                                         args.chunk)
     hps32 = args.steps * nh / el32
     out = {
         "metric": METRIC,
@@ -419,11 +456,13 @@ def main():
     }
     if not args.no_fp64:
         log("slab FP64")
-        el64, st64, e64, _ = time_slab(prob, D.FP64, args.hist, args.steps, args.warmup, args.pipelines, args.chunk)
+        el64, st64, e64, _, b64 = time_slab(prob, D.FP64, args.hist, args.steps, args.warmup, args.pipelines,
+                                            args.chunk, seed0=3000)
         hps64 = args.steps * nh / el64
         out["fp64"] = {"value": hps64, "ms_per_step": 1000.0 * el64 / args.steps, "roofline": roofline(st64, 0)}
         out["fp32_over_fp64"] = hps32 / hps64
         out["energy_per_history_MeV"] = {"fp32": float(np.mean(e32)) / nh, "fp64": float(np.mean(e64)) / nh}
+        out["accuracy_fp32_vs_fp64"] = accuracy_gates(prob, b32, b64)
     out["device_stats_fp32_last_step"] = st32[-1]
     if not args.no_tune:
         log("autotune FP32 on the slab workload")
