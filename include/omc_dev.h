@@ -68,6 +68,15 @@ typedef struct {
                                 nothing and costs a launch per repack;
                                 0 = automatic (the lanes the device holds
                                 resident), < 0 = every launch drains */
+    int     woodcock;        /* photon flights by delta (Woodcock) tracking
+                                where a phantom allows it (uniform voxel
+                                grid, one photon energy grid for its media):
+                                interaction sites come from the same
+                                distribution as with voxel stepping, but
+                                histories are not the reference's step for
+                                step.  0 = automatic (on for the FP32
+                                engine, off for FP64, which keeps its
+                                history-by-history parity), > 0 on, < 0 off */
 } omc_dev_opts;
 
 /* Device counters of one run (all sums over the run). */
@@ -92,6 +101,9 @@ typedef struct {
                                    generation ends), not waiting */
     double  pipe_idle_ms;       /* summed over pipelines: device time between a
                                    generation's end and the next one's start */
+    int64_t woodcock_over;      /* delta-tracking sites whose attenuation
+                                   exceeded the majorant (0 unless the
+                                   majorant is wrong) */
 } omc_dev_stats;
 
 /* Context: device, pipelines (each a stream with its own particle queues),

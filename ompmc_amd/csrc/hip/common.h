@@ -214,6 +214,17 @@ template <typename R> struct DevTables {
 };
 
 // Resident phantom: region r = voxel + 1, region 0 = outside.
+// Delta tracking of photon flights in a phantom with a uniform voxel grid
+// whose media share one photon energy grid: mu[bin] bounds the attenuation
+// coefficient (1/cm) of every voxel at every energy of that bin.
+template <typename R> struct Woodcock {
+    const R *mu;             // [MXGE] majorant; null = voxel stepping only
+    double ge0, ge1;         // lgle = ge1 * log(E) + ge0 for every medium
+    double x0, y0, z0;       // lower corner of the grid
+    R idx, idy, idz;         // inverse voxel sizes
+    int pad;
+};
+
 template <typename R> struct DevPhantom {
     int isize, jsize, ksize, nreg;
     const double *xb, *yb, *zb;
@@ -221,6 +232,7 @@ template <typename R> struct DevPhantom {
     const R *rhof;           // [nreg]
     double ecut[MAXMED + 1]; // [imed + 1]; entry 0 is vacuum / outside
     double pcut[MAXMED + 1];
+    Woodcock<R> wc;
 };
 
 // One source job (a dosxyz field, one beamlet, one serving request).
@@ -314,6 +326,7 @@ struct DevStatus {
     unsigned long long photon_steps, electron_steps;   // transport loop trips
     unsigned long long photon_cross, electron_cross;   // voxel boundaries crossed
     unsigned long long photon_dep, electron_dep;       // ausgab calls
+    unsigned long long wc_over;                        // delta-tracking sites above the majorant
 };
 
 }  // namespace omc

@@ -101,6 +101,12 @@ struct PhantomSlot {
     int nreg = 0;
     int isize = 0, jsize = 0, ksize = 0;
     DevBuf xb, yb, zb, med, rhof, rho;
+    // delta tracking: largest relative density of each medium (-1 = absent)
+    // and the grid, if uniform
+    double rho_max[MAXMED];
+    bool uniform = false;
+    double x0 = 0, y0 = 0, z0 = 0, dx = 0, dy = 0, dz = 0;
+    DevBuf wc;                    // majorant [MXGE]
 };
 
 // Host mirror of a pipeline's small device state, in pinned memory so the
@@ -208,6 +214,11 @@ struct omc_dev_ctx {
     // tables
     bool media = false;
     int nmed = 0;
+    bool woodcock = false;            // photon flights by delta tracking
+    // host copy of the photon attenuation tables, for the majorants:
+    // {gmfp1, gmfp0, cohe1, cohe0} per [imed*MXGE + bin]
+    std::vector<double> h_mfp;
+    double h_ge0[MAXMED] = {}, h_ge1[MAXMED] = {};
     void *small = nullptr, *ph_mfp = nullptr, *ph_br = nullptr, *ph_pmax = nullptr;
     void *el = nullptr, *csda = nullptr, *spin = nullptr, *ms = nullptr;
     // phantoms
@@ -353,6 +364,64 @@ void free_pipe(Pipe &P)
 }  // namespace
 
 // ---------------------------------------------------------------- tables --
+// Majorant of the photon attenuation coefficient over a phantom, per energy
+// bin of the energy grid its media share (delta tracking, see photon_flight):
+// within a bin the tables are linear in log(E), so gmfp and cohfac take their
+// smallest values at an edge of the bin, and mu = rhof / (gmfp * cohfac) is at
+// most rho_max / (min gmfp * min cohfac).  The end bins also cover a bin's
+// width beyond the grid; a 0.1% margin covers FP32 rounding.  A phantom with
+// a non-uniform grid, media on different energy grids or a table that is not
+// positive keeps voxel stepping (null majorant).
+template <typename R> static int build_woodcock(omc_dev_ctx *c, int slot)
+{
+    PhantomSlot &P = c->ph[slot];
+    if (P.wc.p) { dfree(c, P.wc.p, P.wc.bytes); P.wc.p = nullptr; P.wc.bytes = 0; }
+    Woodcock<R> w;
+    memset(&w, 0, sizeof w);
+    int ref = -1;
+    bool ok = P.uniform && c->media;
+    for (int m = 0; ok && m < c->nmed; ++m) {
+        if (P.rho_max[m] < 0) continue;
+        if (ref < 0) ref = m;
+        else if (c->h_ge0[m] != c->h_ge0[ref] || c->h_ge1[m] != c->h_ge1[ref]) ok = false;
+    }
+    std::vector<R> mu(MXGE, (R)1.0E-30);
+    if (ok && ref >= 0) {
+        const double ge0 = c->h_ge0[ref], ge1 = c->h_ge1[ref];
+        if (!(ge1 > 0.0)) ok = false;
+        for (int k = 0; ok && k < MXGE; ++k) {
+            // bin k holds ge1 * log(E) + ge0 in [k + 1, k + 2) (pwlf_bin)
+            double lo = (k + 1 - ge0) / ge1, hi = (k + 2 - ge0) / ge1;
+            const double wbin = hi - lo;
+            if (k == 0) lo -= wbin;
+            if (k == MXGE - 1) hi += wbin;
+            double top = 0.0;
+            for (int m = 0; m < c->nmed; ++m) {
+                if (P.rho_max[m] <= 0) continue;
+                const double *t = &c->h_mfp[4 * ((size_t)m * MXGE + k)];
+                const double amin = std::min(lo * t[0] + t[1], hi * t[0] + t[1]);
+                const double bmin = std::min(lo * t[2] + t[3], hi * t[2] + t[3]);
+                if (!(amin > 0.0 && bmin > 0.0)) { ok = false; break; }
+                top = std::max(top, P.rho_max[m] / (amin * bmin));
+            }
+            mu[k] = (R)std::max(1.001 * top, 1.0E-30);
+        }
+    }
+    if (ok) {
+        P.wc.bytes = sizeof(R) * MXGE;
+        OMC_TRY_DEV(dalloc(c, &P.wc.p, P.wc.bytes));
+        HIPCHK(hipMemcpy(P.wc.p, mu.data(), P.wc.bytes, hipMemcpyHostToDevice));
+        w.mu = (const R *)P.wc.p;
+        w.ge0 = c->h_ge0[ref >= 0 ? ref : 0];
+        w.ge1 = c->h_ge1[ref >= 0 ? ref : 0];
+        w.x0 = P.x0; w.y0 = P.y0; w.z0 = P.z0;
+        w.idx = (R)(1.0 / P.dx); w.idy = (R)(1.0 / P.dy); w.idz = (R)(1.0 / P.dz);
+    }
+    HIPCHK(hipMemcpy((char *)c->d_phantoms + slot * sizeof(DevPhantom<R>) + offsetof(DevPhantom<R>, wc), &w,
+                     sizeof w, hipMemcpyHostToDevice));
+    return OMC_OK;
+}
+
 template <typename R> static int upload_tables(omc_dev_ctx *c, const omc_tables *t)
 {
     const int nmed = t->nmed;
@@ -433,6 +502,14 @@ template <typename R> static int upload_tables(omc_dev_ctx *c, const omc_tables 
     }
     c->media = true;
     c->nmed = nmed;
+    c->h_mfp.resize(G * 4);
+    for (size_t i = 0; i < G; ++i) {
+        c->h_mfp[4 * i] = t->gmfp1[i]; c->h_mfp[4 * i + 1] = t->gmfp0[i];
+        c->h_mfp[4 * i + 2] = t->cohe1[i]; c->h_mfp[4 * i + 3] = t->cohe0[i];
+    }
+    for (int m = 0; m < nmed; ++m) { c->h_ge0[m] = t->ge0[m]; c->h_ge1[m] = t->ge1[m]; }
+    for (int i = 0; i < MAX_PHANTOMS; ++i)
+        if (c->ph[i].used) OMC_TRY_DEV(build_woodcock<R>(c, i));
     return OMC_OK;
 }
 
@@ -519,6 +596,20 @@ template <typename R> static int upload_phantom(omc_dev_ctx *c, const omc_phanto
         }
     }
     for (int m = 1; m <= MAXMED; ++m) { if (ecut[m] < 0) ecut[m] = 0.0; if (pcut[m] < 0) pcut[m] = 0.0; }
+    for (int m = 0; m < MAXMED; ++m) P.rho_max[m] = -1.0;
+    for (int i = 1; i < nreg; ++i)
+        if (med[i] != 0xff) P.rho_max[med[i]] = std::max(P.rho_max[med[i]], (double)rhof[i]);
+    // a uniform grid lets delta tracking find a voxel from a position
+    auto uniform_axis = [](const double *b, int n, double &lo, double &d) {
+        lo = b[0];
+        d = (b[n] - b[0]) / n;
+        if (!(d > 0.0)) return false;
+        for (int i = 0; i < n; ++i)
+            if (std::fabs((b[i + 1] - b[i]) - d) > 1e-6 * d) return false;
+        return true;
+    };
+    P.uniform = uniform_axis(ph->xbounds, ph->isize, P.x0, P.dx) && uniform_axis(ph->ybounds, ph->jsize, P.y0, P.dy) &&
+                uniform_axis(ph->zbounds, ph->ksize, P.z0, P.dz);
     OMC_TRY_DEV(grow_finish_scratch(c, nvox));
     P.nreg = nreg;
     P.isize = ph->isize; P.jsize = ph->jsize; P.ksize = ph->ksize;
@@ -542,6 +633,7 @@ template <typename R> static int upload_phantom(omc_dev_ctx *c, const omc_phanto
     for (int m = 0; m <= MAXMED; ++m) { d.ecut[m] = ecut[m]; d.pcut[m] = pcut[m]; }
     HIPCHK(hipMemcpy((char *)c->d_phantoms + slot * sizeof(DevPhantom<R>), &d, sizeof d, hipMemcpyHostToDevice));
     P.used = true;
+    OMC_TRY_DEV(build_woodcock<R>(c, slot));
     *id = slot;
     return OMC_OK;
 }
@@ -850,6 +942,7 @@ template <typename R> static int launch_generation(omc_dev_ctx *c, int pi)
     A.drain_trips = c->drain_trips;
     A.event_div = c->event_div;
     A.fetch_min = c->fetch_min;
+    A.woodcock = c->woodcock ? 1u : 0u;
     const int max_grid = c->cus * 8;
     P.had_ph = inP > 0;
     if (P.had_ph) {
@@ -1049,6 +1142,7 @@ static void stats_end(omc_dev_ctx *c, omc_dev_stats *out, int64_t histories, dou
         S.photon_deposits += (int64_t)(P.h->status.photon_dep - P.base.photon_dep);
         S.electron_deposits += (int64_t)(P.h->status.electron_dep - P.base.electron_dep);
         S.killed += (int64_t)(P.h->status.killed - P.base.killed);
+        S.woodcock_over += (int64_t)(P.h->status.wc_over - P.base.wc_over);
         S.yields += (int64_t)(P.h->status.yields - P.base.yields);
     }
     S.deposits = S.photon_deposits + S.electron_deposits;
@@ -1154,6 +1248,7 @@ int omc_dev_create(omc_dev_ctx **out, const omc_dev_opts *opts)
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, o.device) == hipSuccess) c->cus = prop.multiProcessorCount;
     c->drain_min = o.drain_min > 0 ? o.drain_min : o.drain_min < 0 ? 0 : drain_min_default(c->cus, c->precision);
+    c->woodcock = o.woodcock > 0 || (o.woodcock == 0 && o.precision == OMC_PREC_FP32);
     size_t freeb = 0, totalb = 0;
     (void)hipMemGetInfo(&freeb, &totalb);
     c->budget = o.hbm_budget > 0 ? o.hbm_budget : (int64_t)(0.9 * (double)freeb);
@@ -1184,7 +1279,7 @@ int omc_dev_destroy(omc_dev_ctx *c)
                      c->d_jobs, c->d_phantoms, c->fin_sums };
     for (void *p : ptrs) if (p) (void)hipFree(p);
     for (auto &P : c->ph) {
-        DevBuf *bufs[] = { &P.xb, &P.yb, &P.zb, &P.med, &P.rhof, &P.rho };
+        DevBuf *bufs[] = { &P.xb, &P.yb, &P.zb, &P.med, &P.rhof, &P.rho, &P.wc };
         for (auto *b : bufs) if (b->p) (void)hipFree(b->p);
     }
     for (auto &kv : c->tickets) delete kv.second;
@@ -1219,8 +1314,8 @@ int omc_dev_release_phantom(omc_dev_ctx *c, int id)
     for (auto *w : c->waiting)
         if (w->phantom == id) return fail(OMC_EARG, "phantom %d is in use", id);
     PhantomSlot &P = c->ph[id];
-    DevBuf *bufs[] = { &P.xb, &P.yb, &P.zb, &P.med, &P.rhof, &P.rho };
-    for (auto *b : bufs) { dfree(c, b->p, b->bytes); b->p = nullptr; b->bytes = 0; }
+    DevBuf *bufs[] = { &P.xb, &P.yb, &P.zb, &P.med, &P.rhof, &P.rho, &P.wc };
+    for (auto *b : bufs) { if (b->p) dfree(c, b->p, b->bytes); b->p = nullptr; b->bytes = 0; }
     P.used = false;
     return OMC_OK;
 }
@@ -1427,6 +1522,7 @@ int omc_dev_tune(omc_dev_ctx *c, const omc_source_dosxyz *src, const omc_dev_run
     best->drain_lanes = (int)c->drain_lanes;
     best->drain_trips = (int)c->drain_trips;
     best->drain_min = c->drain_min > 0 ? c->drain_min : -1;
+    best->woodcock = c->woodcock ? 1 : -1;
     best->event_div = (int)c->event_div;
     best->fetch_min = (int)c->fetch_min;
     *hps = bv;

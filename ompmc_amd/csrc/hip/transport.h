@@ -37,6 +37,7 @@ template <typename R> struct KArgs {
     uint32_t drain_trips;    // ...if they have taken this many trips here
     uint32_t event_div;      // photon events run once 1/event_div of the live lanes wait
     uint32_t fetch_min;      // free lanes take new particles once this many are free
+    uint32_t woodcock;       // photon flights by delta tracking where the phantom allows
 };
 
 // Per-lane geometry / scoring context for the particle the lane holds.  The
@@ -452,6 +453,68 @@ __device__ __forceinline__ bool photon_step(const KArgs<R> &A, Particle<R> &p, P
             s.phase = 2;
             return true;
         }
+    }
+    return true;
+}
+
+// Photon flight by delta (Woodcock) tracking, in place of photon_step for a
+// photon that is not splitting, in a phantom with a uniform grid whose media
+// share one energy grid.  Photons deposit nothing in flight, so only the
+// interaction sites matter: the flight runs dpmfp mean free paths of the
+// majorant mu_max(E) >= mu(E, voxel), and at the site a real interaction
+// happens with probability mu/mu_max (otherwise a fresh exponential flight
+// follows).  The sites have the distribution voxel stepping gives, without a
+// stop at every voxel boundary: one medium and density lookup per site.
+// At a real site the lane holds the voxel's medium, density and table values
+// as photon_step would, and the interaction event follows.  A site outside
+// the grid ends the photon (it left the phantom).
+#ifndef OMC_WOODCOCK_SITES
+#define OMC_WOODCOCK_SITES 4
+#endif
+template <typename R>
+__device__ __forceinline__ bool photon_flight(const KArgs<R> &A, Particle<R> &p, PhotonLane<R> &s, LaneGeo<R> &G,
+                                              uint32_t &nsteps, uint32_t &ncross)
+{
+    const DevTables<R> &T = A.T;
+    const DevPhantom<R> &g = *G.g;
+    const Woodcock<R> &W = g.wc;
+    const R mu_max = glob(W.mu)[pwlf_bin(s.gle, (R)W.ge1, (R)W.ge0, MXGE)];
+    for (int k = 0; k < OMC_WOODCOCK_SITES; ++k) {
+        nsteps++;
+        advance<R>(p.x, p.y, p.z, p.u, p.v, p.w, s.dpmfp / mu_max);
+        const R fx = (p.x - (R)W.x0) * W.idx, fy = (p.y - (R)W.y0) * W.idy, fz = (p.z - (R)W.z0) * W.idz;
+        if (!(fx >= (R)0 && fy >= (R)0 && fz >= (R)0 && fx <= (R)g.isize && fy <= (R)g.jsize &&
+              fz <= (R)g.ksize)) {                                     // left the phantom: discard
+            if (G.ledger) atomicAdd((double *)(G.ledger + 2), (double)p.wt * (double)p.e);
+            return false;
+        }
+        const int ix = min((int)fx, g.isize - 1), iy = min((int)fy, g.jsize - 1), iz = min((int)fz, g.ksize - 1);
+        const int ir = 1 + ix + iy * g.isize + iz * g.isize * g.jsize;
+        ncross++;
+        p.ir = ir;
+        const int m = G.medium(ir);
+        s.imed = m;
+        if (m >= 0) {
+            const R rho = G.rho(ir);
+            if (m != s.cmed) {
+                const MediumScalars &M = T.small->med[m];
+                s.lgle = pwlf_bin(s.gle, M.ge1, M.ge0, MXGE);
+                R4<R> c = T.ph_mfp[m * MXGE + s.lgle];
+                s.gmfpr0 = s.gle * c.a1 + c.a0;
+                s.cohfac = s.gle * c.b1 + c.b0;
+                s.cmed = m;
+            }
+            const R mu = rho / (s.gmfpr0 * s.cohfac);
+            if (mu > mu_max) atomicAdd(&A.status->wc_over, 1ull);
+            if (urand<R>(p.rng) * mu_max <= mu) {                     // real interaction
+                s.rhof = rho;
+                s.gmfp = s.gmfpr0 / rho * s.cohfac;
+                s.dpmfp = (R)0.0;
+                s.phase = 2;
+                return true;
+            }
+        }
+        s.dpmfp = -m_log((R)1.0 - urand<R>(p.rng));
     }
     return true;
 }
@@ -1003,7 +1066,12 @@ __global__ void __launch_bounds__(256, Waves<R>::photon) k_photon(KArgs<R> A)
         const bool events = nev * A.event_div >= nlive || nev == nlive;
         if (alive && (s.phase == 1 || events)) {
             if (OMC_RNG_DEPTH_P) p.rng.top_up(OMC_RNG_DEPTH_P);
-            alive = s.phase == 1 ? photon_step<R>(A, p, s, G, nsteps, ncross) : photon_event<R>(A, p, s, G, sc);
+            if (s.phase != 1)
+                alive = photon_event<R>(A, p, s, G, sc);
+            else if (A.woodcock && G.g->wc.mu && s.split == 0u)
+                alive = photon_flight<R>(A, p, s, G, nsteps, ncross);
+            else
+                alive = photon_step<R>(A, p, s, G, nsteps, ncross);
             ++trips;
             ++here;
             if (alive && trips > MAX_ITERS) {           // watchdog: no particle runs forever

@@ -1,0 +1,110 @@
+"""Photon flights by delta (Woodcock) tracking.
+
+The reference steps a photon voxel by voxel, decrementing its mean free
+paths (ompmc.c:1951-2019).  Photons deposit nothing in flight, so sampling
+the interaction sites against a majorant attenuation coefficient and
+accepting each with probability mu/mu_max gives the same dose
+distribution with a handful of sites per flight.  The FP32 engine does
+this by default where a phantom allows it; the FP64 engine keeps voxel
+stepping (history-by-history parity with the oracle) unless asked.
+
+Checks: the dose agrees with voxel stepping within statistics (voxels
+within 2 sigma, integral energy within 4 standard errors of the per-history
+ledger), no site ever exceeds the majorant, energy is conserved history by
+history, and a phantom the tracking cannot serve (non-uniform grid) falls
+back to voxel stepping exactly."""
+import numpy as np
+import pytest
+
+from cases import write_case
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def D():
+    from ompmc_amd import device
+    return device
+
+
+def _run(D, prob, precision, woodcock, seed, ledger=False):
+    with D.Engine(precision, woodcock=woodcock) as eng:
+        eng.upload_media(prob.tab)
+        pid = eng.upload_phantom(prob.ph, prob.reg)
+        return eng.run_dosxyz(pid, prob.src, prob.nperbatch, prob.nbatch, seed=seed, nreg=prob.reg.nreg,
+                              ledger=ledger)
+
+
+def _mv(s, s2, nb):
+    m = s / nb
+    return m, np.sqrt(np.maximum(s2 / nb - m * m, 0) / (nb - 1))
+
+
+@pytest.mark.parametrize("case,precision", [("slab6MV", 0), ("patient6MV", 0), ("patient6MV", 1)])
+def test_delta_tracking_matches_voxel_stepping(oracle, D, tmp_path, case, precision):
+    stem = write_case(case, tmp_path, ncase=200000)
+    inp = tmp_path / f"{case}.inp"
+    inp.write_text(inp.read_text().replace("nbatch = 10", "nbatch = 50"))
+    prob = oracle.load_problem(stem)
+    try:
+        nb, nh = prob.nbatch, prob.nhist
+        sv, sv2, stv, lv = _run(D, prob, 0, -1, (31, 7), ledger=True)               # voxel stepping, FP64
+        sw, sw2, stw, lw = _run(D, prob, precision, 1, (32, 7), ledger=True)       # delta tracking
+    finally:
+        oracle.free_problem(prob)
+    assert stw.woodcock_over == 0 and stw.killed == 0
+    # a few sites per flight instead of a stop at every voxel boundary
+    assert stw.photon_steps < 0.25 * stv.photon_steps, (stw.photon_steps, stv.photon_steps)
+    mv, ev = _mv(sv, sv2, nb)
+    mw, ew = _mv(sw, sw2, nb)
+    sel = np.where(mv[1:] > 0.1 * mv[1:].max())[0] + 1
+    z = (mw[sel] - mv[sel]) / np.sqrt(ev[sel] ** 2 + ew[sel] ** 2 + 1e-300)
+    frac = (np.abs(z) < 2).mean()
+    assert len(sel) > 50 and frac >= 0.93, f"{case}: {frac:.3f} of {len(sel)} voxels within 2 sigma"
+    # integral: mean energy deposited per history, independent histories
+    dv, dw = lv.reshape(-1, 3)[:, 0], lw.reshape(-1, 3)[:, 0]
+    se = np.sqrt(dv.var() / nh + dw.var() / nh)
+    assert abs(dw.mean() - dv.mean()) < 4 * se, (dw.mean(), dv.mean(), se)
+
+
+@pytest.mark.parametrize("precision,tol", [(0, 1e-12), (1, 2e-6)])
+def test_delta_tracking_conserves_energy_per_history(oracle, D, tmp_path, precision, tol):
+    """Every 2 MeV history's energy is deposited in the phantom, in region 0
+    or leaves it (ledger columns), and the ledger's deposits are the
+    tallies."""
+    stem = write_case("water2MeV", tmp_path, ncase=20000)
+    prob = oracle.load_problem(stem)
+    try:
+        s, _, st, led = _run(D, prob, precision, 1, (5, 9), ledger=True)
+    finally:
+        oracle.free_problem(prob)
+    led = led.reshape(-1, 3)
+    assert st.woodcock_over == 0 and st.killed == 0
+    assert st.photon_steps < 10 * prob.nhist
+    assert np.allclose(led.sum(1), 2.0, rtol=tol, atol=tol)
+    assert abs(led[:, :2].sum() - s.sum()) < 1e-9 * s.sum()
+
+
+def test_non_uniform_grid_falls_back_to_voxel_stepping(D, tmp_path):
+    """A phantom whose voxels differ in size cannot be tracked by position:
+    asking for delta tracking there must give exactly the voxel-stepping
+    histories (FP64, same streams)."""
+    from ompmc_amd import configs as cfg
+    from ompmc_amd import problem as P
+    ph = cfg.water_cube(10, 5.0)
+    ph.xb = np.concatenate([ph.xb[:5], ph.xb[5:] + np.linspace(0.0, 0.3, len(ph.xb) - 5)])
+    ph.med[:, :, 6:] = 1
+    ph.rho[:, :, 6:] = 1.4
+    pf = ph.write_egsphant(tmp_path / "nonuni.egsphant")
+    items = cfg.dosxyz_inp_items(tmp_path, pf, ncase=4000, mono=2.0, field=(-1, 1, -1, 1))
+    cfg.write_inp(tmp_path / "nonuni", items)
+    prob = P.load_problem(tmp_path / "nonuni")
+    try:
+        so, so2, sto, _ = _run(D, prob, 0, -1, (3, 3))
+        sw, sw2, stw, _ = _run(D, prob, 0, 1, (3, 3))
+    finally:
+        P.free_problem(prob)
+    assert stw.photon_steps == sto.photon_steps
+    scale = np.abs(so).max()
+    assert scale > 0
+    assert np.allclose(sw, so, rtol=0, atol=1e-12 * scale)
