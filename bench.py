@@ -451,7 +451,9 @@ def main():
         "data": "synthetic (phantoms, spectrum and beamlets generated in-process)",
         "config": {"workload": "slab phantom 64^3 @ 0.25 cm (water/lung 0.26/bone 1.85/water), synthetic 6 MV, "
                                "10x10 cm2 at SSD 100, ECUT 0.7 MeV, PCUT 0.01 MeV; FP32 transport, FP64 tallies",
-                   "histories_per_step": nh, "nbatch": 10},
+                   "histories_per_step": nh, "nbatch": 10,
+                   "photon_flights": {"fp32": "delta tracking (engine default)",
+                                      "fp64": "voxel stepping, as the reference (engine default)"}},
         "roofline": roofline(st32, 1),
     }
     if not args.no_fp64:
