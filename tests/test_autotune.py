@@ -15,7 +15,7 @@ CANDIDATES = {
     "drain_lanes": {16, 32, 48},
     "drain_trips": {8, 16, 32},
     "event_div": {1, 2, 4},
-    "fetch_min": {4, 16, 32},
+    "fetch_min": {1, 4, 8, 16},
 }
 
 
