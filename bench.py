@@ -221,9 +221,9 @@ def roofline(stats, precision):
 
 
 # --------------------------------------------------------- beamlet batch --
-def beamlet_batch(prob, src, nhist, precision, nbatch=10):
+def beamlet_batch(prob, src, nhist, precision, nbatch=10, engine=None):
     from ompmc_amd import device as D
-    with D.Engine(precision) as eng:
+    with D.Engine(precision, **(engine or {})) as eng:
         eng.upload_media(prob.tab)
         pid = eng.upload_phantom(prob.ph, prob.reg)
         small = np.minimum(nhist, 2000)
@@ -476,12 +476,18 @@ def main():
             for name, prec in (("fp32", D.FP32), ("fp64", D.FP64)):
                 log(f"beamlets {name}: {src.nbeamlets} beamlets, {int(bl_nh.sum())} histories")
                 res[name], _ = beamlet_batch(pprob, src, bl_nh, prec)
+            log("beamlets fp32, photon flights stepped voxel by voxel as in the reference")
+            res["fp32_voxel_stepping"], _ = beamlet_batch(pprob, src, bl_nh, D.FP32, engine={"woodcock": -1})
             log(f"beamlets plain fp64: {src.nbeamlets} jobs in turn")
             res["plain_fp64"] = beamlet_plain(pprob, src, bl_nh, D.FP64)
             res["fp32_over_fp64"] = res["fp32"]["histories_per_s"] / res["fp64"]["histories_per_s"]
             res["fp32_batched_over_plain_fp64"] = res["fp32"]["histories_per_s"] / res["plain_fp64"]["histories_per_s"]
+            res["fp32_voxel_stepping_batched_over_plain_fp64"] = (res["fp32_voxel_stepping"]["histories_per_s"] /
+                                                                  res["plain_fp64"]["histories_per_s"])
             res["workload"] = ("patient 128x128x96 @ 0.25 cm, 400 beamlets 0.5x0.5 cm2 at gantry 0, histories "
-                               "log-uniform [1e4, 1e6] per beamlet, nbatch 10, sparse Dij at 1% of beamlet max")
+                               "log-uniform [1e4, 1e6] per beamlet, nbatch 10, sparse Dij at 1% of beamlet max; "
+                               "fp32 = delta-tracked photon flights (engine default), fp32_voxel_stepping and the "
+                               "fp64 rows step photons voxel by voxel as the reference does")
             out["beamlets"] = res
         if not args.no_latency:
             out["latency"] = latency_section(pprob, src)

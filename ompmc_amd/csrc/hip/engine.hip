@@ -828,19 +828,23 @@ static int finish_deliver(omc_dev_ctx *c, int j)
 }
 
 // Advances the collection of finished jobs without blocking.  Returns the
-// number of jobs delivered, or an error (the failing job is ended with it).
-static int pump_finish(omc_dev_ctx *c)
+// number of jobs delivered, or an error (the failing job is ended with it);
+// *worked tells whether any stage advanced (otherwise it only polled).
+static int pump_finish(omc_dev_ctx *c, bool *worked)
 {
     int delivered = 0;
+    *worked = false;
     while (!c->finishing.empty()) {
         const int j = c->finishing.front();
         JobRec &J = c->jobs[j];
         int rc = OMC_OK;
         if (J.fin == FIN_QUEUED) {
             rc = finish_fold(c, j);
+            *worked = true;
         } else {
             const hipError_t e = hipEventQuery(c->ev_fin);
             if (e == hipErrorNotReady) return delivered;
+            *worked = true;
             if (e != hipSuccess) rc = fail(OMC_EDEVICE, "finishing: %s", hipGetErrorString(e));
             else if (J.fin == FIN_FOLD && J.w->sink && J.w->sink->sparse) rc = finish_column(c, j);
             else {
@@ -870,7 +874,8 @@ static int drain_finish(omc_dev_ctx *c)
 {
     while (!c->finishing.empty()) {
         HIPCHK(hipStreamSynchronize(c->aux));
-        const int r = pump_finish(c);
+        bool worked;
+        const int r = pump_finish(c, &worked);
         if (r < 0) return r;
     }
     return OMC_OK;
@@ -1088,8 +1093,11 @@ template <typename R> static int turn(omc_dev_ctx *c)
     for (;;) {
         if (!c->finishing.empty()) {
             auto h1 = std::chrono::steady_clock::now();
-            const int d = pump_finish(c);
-            c->acc.host_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - h1).count();
+            bool worked;
+            const int d = pump_finish(c, &worked);
+            if (worked)      // polling a stage still in flight is waiting, not work
+                c->acc.host_ms +=
+                    std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - h1).count();
             if (d < 0) { reset_engine(c, d); return d; }
             if (d > 0) return 1;
         }
