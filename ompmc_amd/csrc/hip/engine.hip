@@ -137,6 +137,7 @@ struct Pipe {
     bool inflight = false;
     bool had_src = false, had_ph = false, had_el = false;
     uint32_t injected = 0;        // primaries injected by the generation in flight
+    uint32_t in_e = 0;            // charged particles in its queue at launch (known to the host)
     std::vector<int> open;        // jobs that may hold particles here
     DevStatus base;               // status at the start of the current run
 };
@@ -937,38 +938,46 @@ template <typename R> static int launch_generation(omc_dev_ctx *c, int pi)
                           P.d_status, s);
         HIPCHK(hipEventRecord(P.ev_src1, s));
     }
+    // The photon kernel appends the electrons it sets in motion to the
+    // current charged-particle queue, whose count the electron kernel reads
+    // when it starts, after the photon kernel on the same stream: a
+    // photon-to-electron hand-over costs no generation.  Everything the
+    // electron kernel makes, and every photon, goes to the next queues.
     KArgs<R> A;
     A.T = tables_view<R>(c);
     A.phantoms = (const DevPhantom<R> *)c->d_phantoms;
     A.jobs = c->d_jobs;
     A.qp = queue_view<R>(c, P, nxt);
-    A.qe = queue_view<R>(c, P, 2 + nxt);
     A.status = P.d_status;
     A.yield_trips = c->yield_trips;
-    A.drain_lanes = c->drain_lanes;     // set per launch below
+    A.drain_lanes = c->drain_lanes;
     A.drain_trips = c->drain_trips;
+    A.drain_min = (uint32_t)std::min<int64_t>(c->drain_min, UINT32_MAX);
     A.event_div = c->event_div;
     A.fetch_min = c->fetch_min;
     A.woodcock = c->woodcock ? 1u : 0u;
     const int max_grid = c->cus * 8;
     P.had_ph = inP > 0;
+    P.in_e = inE;
     if (P.had_ph) {
         A.qin = qpc;
-        A.drain_lanes = (int64_t)inP >= c->drain_min ? c->drain_lanes : 0u;
+        A.qe = qec;
         A.in_count = P.d_counts + P.cur;
         A.fetch = P.d_counts + 4;
         HIPCHK(hipEventRecord(P.ev_ph0, s));
         Launch<R>::photon(A, (int)std::min<int64_t>(max_grid, ((int64_t)inP + 255) / 256), s);
         HIPCHK(hipEventRecord(P.ev_ph1, s));
     }
-    P.had_el = inE > 0;
+    P.had_el = inE > 0 || inP > 0;
     if (P.had_el) {
+        // the persistent grid covers the known electrons and about two per
+        // photon; lanes fetch until the queue's final count either way
         A.qin = qec;
-        A.drain_lanes = (int64_t)inE >= c->drain_min ? c->drain_lanes : 0u;
+        A.qe = queue_view<R>(c, P, 2 + nxt);
         A.in_count = P.d_counts + 2 + P.cur;
         A.fetch = P.d_counts + 5;
         HIPCHK(hipEventRecord(P.ev_el0, s));
-        Launch<R>::electron(A, (int)std::min<int64_t>(max_grid, ((int64_t)inE + 255) / 256), s);
+        Launch<R>::electron(A, (int)std::min<int64_t>(max_grid, ((int64_t)inE + 2 * (int64_t)inP + 255) / 256), s);
         HIPCHK(hipEventRecord(P.ev_el1, s));
     }
     HIPCHK(hipGetLastError());
@@ -1017,7 +1026,9 @@ static int complete_generation(omc_dev_ctx *c, int pi)
     }
     S.generations++;
     S.photons_pushed += P.h->counts[P.cur];
-    S.electrons_pushed += P.h->counts[2 + P.cur];
+    // electrons into the next queue, and those the photon kernel appended
+    // to the queue this generation consumed
+    S.electrons_pushed += P.h->counts[2 + P.cur] + (std::min(P.h->counts[2 + (P.cur ^ 1)], c->qcap) - P.in_e);
     if (P.h->status.overflow)
         return fail(OMC_EOVERFLOW, "particle queue overflow (capacity %u per queue)", c->qcap);
     // jobs with no particle left in this pipeline

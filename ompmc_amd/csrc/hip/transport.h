@@ -34,7 +34,8 @@ template <typename R> struct KArgs {
     uint32_t yield_trips;    // loop trips a particle may take per launch
     uint32_t drain_lanes;    // once the input is used up, a wave with at most
                              // this many live lanes hands its particles on...
-    uint32_t drain_trips;    // ...if they have taken this many trips here
+    uint32_t drain_trips;    // ...if they have taken this many trips here...
+    uint32_t drain_min;      // ...and the launch's input holds at least this many
     uint32_t event_div;      // photon events run once 1/event_div of the live lanes wait
     uint32_t fetch_min;      // free lanes take new particles once this many are free
     uint32_t woodcock;       // photon flights by delta tracking where the phantom allows
@@ -1058,7 +1059,7 @@ __global__ void __launch_bounds__(256, Waves<R>::photon) k_photon(KArgs<R> A)
         // input used up and the wave mostly empty: hand the rest on, the
         // next launch packs them into full waves
         const uint32_t nlive = (uint32_t)__popcll(live);
-        const bool drain = __ballot(exhausted) != 0 && nlive <= A.drain_lanes;
+        const bool drain = in_end >= A.drain_min && __ballot(exhausted) != 0 && nlive <= A.drain_lanes;
         // events (interaction, entry) run for all waiting lanes of the
         // wavefront together, once enough of the live lanes wait for one or
         // none can step; until then the waiting lanes sit out the steps
@@ -1129,7 +1130,8 @@ __global__ void __launch_bounds__(256, Waves<R>::electron) k_electron(KArgs<R> A
         }
         const unsigned long long live = __ballot(alive);
         if (live == 0) break;
-        const bool drain = __ballot(exhausted) != 0 && (uint32_t)__popcll(live) <= A.drain_lanes;
+        const bool drain =
+            in_end >= A.drain_min && __ballot(exhausted) != 0 && (uint32_t)__popcll(live) <= A.drain_lanes;
         if (alive) {
             p.rng.top_up(OMC_RNG_DEPTH_E);
             int r = electron_iteration<R>(A, p, s, G, sc, nsteps, ncross);
