@@ -45,9 +45,10 @@ def drain(eng, tickets, nreg, max_steps=200000):
     raise AssertionError("requests did not finish")
 
 
-def test_request_result_same_alone_and_in_mix(D, problems):
+@pytest.mark.parametrize("precision", [0, 1])
+def test_request_result_same_alone_and_in_mix(D, problems, precision):
     pp, src = problems
-    with D.Engine(0) as eng:
+    with D.Engine(precision) as eng:
         eng.upload_media(pp.tab)
         pid = eng.upload_phantom(pp.ph, pp.reg)
         nreg = pp.reg.nreg

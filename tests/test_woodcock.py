@@ -108,3 +108,39 @@ def test_non_uniform_grid_falls_back_to_voxel_stepping(D, tmp_path):
     scale = np.abs(so).max()
     assert scale > 0
     assert np.allclose(sw, so, rtol=0, atol=1e-12 * scale)
+
+
+def test_delta_tracking_beamlets_match_voxel_stepping(oracle, D, tmp_path):
+    """matRad beamlets (primaries start on the phantom's surface, coming in
+    from two gantry angles): per-beamlet dose of the FP32 engine with delta
+    tracking against FP64 voxel stepping, voxels above 10% of each
+    beamlet's maximum within 2 sigma."""
+    from ompmc_amd import _native as N, configs as cfg
+    ph = cfg.patient_phantom(24, 20, 16, 1.0)
+    pf = ph.write_egsphant(tmp_path / "phantom.egsphant")
+    spec = cfg.write_spectrum(tmp_path / "6mv.spectrum")
+    cfg.write_inp(tmp_path / "bl", cfg.dosxyz_inp_items(tmp_path, pf, ncase=100, spectrum_file=spec))
+    prob = oracle.load_problem(tmp_path / "bl")
+    try:
+        bs = cfg.beamlets(gantry_deg=(0.0, 90.0), field=2.0, bixel=1.0, sad=100.0)
+        src = N.beamlet_source(prob.src.spec, bs.ibeam, bs.src, bs.corner, bs.side1, bs.side2)
+        nh, nb = [40000] * src.nbeamlets, 20
+        out = {}
+        for prec, wc, seed in ((0, -1, (41, 1)), (1, 0, (42, 1))):
+            with D.Engine(prec, woodcock=wc) as eng:
+                eng.upload_media(prob.tab)
+                pid = eng.upload_phantom(prob.ph, prob.reg)
+                s, s2, st = eng.run_beamlets(pid, src, nh, nb, seed=seed)
+                out[prec] = (s, s2, st)
+    finally:
+        oracle.free_problem(prob)
+    assert out[1][2].woodcock_over == 0 and out[1][2].killed == 0
+    inside, total = 0, 0
+    for b in range(src.nbeamlets):
+        mv, ev = _mv(out[0][0][b], out[0][1][b], nb)
+        mw, ew = _mv(out[1][0][b], out[1][1][b], nb)
+        sel = np.where(mv[1:] > 0.1 * mv[1:].max())[0] + 1
+        z = (mw[sel] - mv[sel]) / np.sqrt(ev[sel] ** 2 + ew[sel] ** 2 + 1e-300)
+        inside += int((np.abs(z) < 2).sum())
+        total += len(sel)
+    assert total > 100 and inside / total >= 0.93, f"{inside / total:.3f} of {total} voxels within 2 sigma"
