@@ -1316,6 +1316,13 @@ int omc_dev_upload_media(omc_dev_ctx *c, const omc_tables *t)
     if (!c || !t) return fail(OMC_EARG, "null argument");
     if (t->nmed < 1 || t->nmed > OMC_MXMED) return fail(OMC_ERANGE, "nmed %d", t->nmed);
     if (any_running(c) || !c->waiting.empty()) return fail(OMC_EARG, "media cannot change while work is queued");
+    // the tables must still cover every medium a resident phantom uses
+    for (int i = 0; i < MAX_PHANTOMS; ++i) {
+        if (!c->ph[i].used) continue;
+        for (int m = t->nmed; m < MAXMED; ++m)
+            if (c->ph[i].rho_max[m] >= 0)
+                return fail(OMC_EARG, "resident phantom %d uses medium %d, new tables have %d", i, m, t->nmed);
+    }
     HIPCHK(hipSetDevice(c->device));
     return c->precision == OMC_PREC_FP64 ? upload_tables<double>(c, t) : upload_tables<float>(c, t);
 }

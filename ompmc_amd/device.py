@@ -128,6 +128,11 @@ class Engine:
         self.nreg[pid.value] = regions.nreg
         return pid.value
 
+    def release_phantom(self, pid):
+        """omc_dev_release_phantom: frees a resident phantom no work uses."""
+        check(lib().omc_dev_release_phantom(self.ctx, int(pid)))
+        self.nreg.pop(pid, None)
+
     def run_dosxyz(self, pid, src, nperbatch, nbatch, seed=(97, 33), hist0=0, nreg=None, ledger=False, nsplit=1):
         stats = DevStats()
         led = np.zeros(3 * nperbatch * nbatch) if ledger else None

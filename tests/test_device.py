@@ -345,3 +345,28 @@ def test_device_counters_equal_oracle_counts(oracle, D, tmp_path, case):
     for dev, ora in ((st.deposits, cnt.n_dep), (st.photon_crossings, cnt.n_cross_photon),
                      (st.electron_crossings, cnt.n_cross_charged)):
         assert abs(dev - ora) <= 1e-3 * ora + 2, (dev, ora)
+
+
+def test_media_upload_must_cover_resident_phantoms(oracle, D, tmp_path):
+    """New tables with fewer media than a resident phantom uses would let
+    the kernels index past them: the upload is refused until the phantom
+    is released."""
+    pp = oracle.load_problem(write_case("patient6MV", tmp_path / "p", ncase=100))
+    wp = oracle.load_problem(write_case("water2MeV", tmp_path / "w", ncase=100))
+    try:
+        assert wp.tab.nmed < pp.tab.nmed
+        with D.Engine(1) as eng:
+            eng.upload_media(pp.tab)
+            pid = eng.upload_phantom(pp.ph, pp.reg)
+            with pytest.raises(RuntimeError, match="resident phantom"):
+                eng.upload_media(wp.tab)
+            s, _, st, _ = eng.run_dosxyz(pid, pp.src, pp.nperbatch, pp.nbatch, nreg=pp.reg.nreg)
+            assert st.killed == 0 and s[1:].sum() > 0
+            eng.release_phantom(pid)
+            eng.upload_media(wp.tab)
+            wid = eng.upload_phantom(wp.ph, wp.reg)
+            s, _, st, _ = eng.run_dosxyz(wid, wp.src, wp.nperbatch, wp.nbatch, nreg=wp.reg.nreg)
+            assert st.killed == 0 and s[1:].sum() > 0
+    finally:
+        oracle.free_problem(pp)
+        oracle.free_problem(wp)
