@@ -144,3 +144,37 @@ def test_delta_tracking_beamlets_match_voxel_stepping(oracle, D, tmp_path):
         inside += int((np.abs(z) < 2).sum())
         total += len(sel)
     assert total > 100 and inside / total >= 0.93, f"{inside / total:.3f} of {total} voxels within 2 sigma"
+
+
+def test_delta_tracking_through_vacuum_voxels(D, tmp_path):
+    """A vacuum slab inside the grid (no attenuation, every site there is
+    fictitious) and a denser region behind it: the same dose as voxel
+    stepping within statistics, energy conserved per history."""
+    from ompmc_amd import configs as cfg
+    from ompmc_amd import problem as P
+    ph = cfg.water_cube(12, 6.0)
+    ph.med[:, :, 4:6] = 0
+    ph.rho[:, :, 4:6] = 0.0
+    ph.rho[:, :, 8:] = 1.8
+    pf = ph.write_egsphant(tmp_path / "vac.egsphant")
+    items = cfg.dosxyz_inp_items(tmp_path, pf, ncase=200000, nbatch=20, mono=3.0, field=(-1, 1, -1, 1))
+    cfg.write_inp(tmp_path / "vac", items)
+    prob = P.load_problem(tmp_path / "vac")
+    try:
+        nb = prob.nbatch
+        sv, sv2, stv, lv = _run(D, prob, 0, -1, (51, 2), ledger=True)
+        sw, sw2, stw, lw = _run(D, prob, 1, 1, (52, 2), ledger=True)
+        nh = prob.nperbatch * prob.nbatch
+    finally:
+        P.free_problem(prob)
+    assert stw.woodcock_over == 0 and stw.killed == 0
+    assert np.allclose(lw.reshape(-1, 3).sum(1), 3.0, rtol=2e-6, atol=2e-6)
+    mv, ev = _mv(sv, sv2, nb)
+    mw, ew = _mv(sw, sw2, nb)
+    sel = np.where(mv[1:] > 0.1 * mv[1:].max())[0] + 1
+    z = (mw[sel] - mv[sel]) / np.sqrt(ev[sel] ** 2 + ew[sel] ** 2 + 1e-300)
+    frac = (np.abs(z) < 2).mean()
+    assert len(sel) > 50 and frac >= 0.93, f"{frac:.3f} of {len(sel)} voxels within 2 sigma"
+    dv, dw = lv.reshape(-1, 3)[:, 0], lw.reshape(-1, 3)[:, 0]
+    se = np.sqrt(dv.var() / nh + dw.var() / nh)
+    assert abs(dw.mean() - dv.mean()) < 4 * se, (dw.mean(), dv.mean(), se)
