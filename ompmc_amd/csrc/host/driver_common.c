@@ -26,6 +26,9 @@ void drv_parse_args(drv_args *a, int argc, char **argv)
     memset(a, 0, sizeof *a);
     const char *env = getenv("OMC_PRECISION");
     a->precision = (env && !strcmp(env, "fp32")) ? OMC_PREC_FP32 : OMC_PREC_FP64;
+    const char *wc = getenv("OMC_WOODCOCK");
+    if (wc && (!strcmp(wc, "0") || !strcmp(wc, "off"))) a->woodcock = -1;
+    else if (wc && (!strcmp(wc, "1") || !strcmp(wc, "on"))) a->woodcock = 1;
     static int verbose_flag = 0;
     enum { OPT_FP32 = 1000, OPT_FP64, OPT_DEVICE };
     static struct option opts[] = {
@@ -79,13 +82,16 @@ omc_dev_ctx *drv_open_device(const drv_args *a, const omc_tables *tab)
     memset(&o, 0, sizeof o);
     o.device = a->device;
     o.precision = a->precision;
+    o.woodcock = a->woodcock;
     omc_dev_ctx *ctx = NULL;
     int rc = omc_dev_create(&ctx, &o);
     if (rc) drv_die("device set-up", rc, 1);
     rc = omc_dev_upload_media(ctx, tab);
     if (rc) drv_die("media upload", rc, 1);
-    printf("Transport on HIP device %d, %s\n", a->device,
-           a->precision == OMC_PREC_FP32 ? "FP32 transport with FP64 tallies" : "FP64 transport");
+    const int delta = a->woodcock > 0 || (a->woodcock == 0 && a->precision == OMC_PREC_FP32);
+    printf("Transport on HIP device %d, %s, photon flights by %s\n", a->device,
+           a->precision == OMC_PREC_FP32 ? "FP32 transport with FP64 tallies" : "FP64 transport",
+           delta ? "delta tracking" : "voxel stepping");
     return ctx;
 }
 

@@ -14,6 +14,8 @@ typedef struct {
     int verbose;            /* --verbose / --brief */
     int precision;          /* --fp32 / --fp64 (default from OMC_PRECISION, else FP64) */
     int device;             /* --device N */
+    int woodcock;           /* OMC_WOODCOCK: 0/off voxel-stepped photon flights, 1/on delta
+                               tracking; unset = the engine's default (on for FP32 only) */
 } drv_args;
 
 /* getopt_long over the reference's options (omc_dosxyz.c:1078-1150) plus

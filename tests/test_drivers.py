@@ -110,3 +110,22 @@ def test_matrad_end_to_end_matches_device_api(oracle, tmp_path):
     assert "relative dose threshold" in items
     assert np.array_equal(jcs, j2) and np.array_equal(irs, i2)
     assert np.allclose(vals, v2, rtol=1e-12, atol=0)
+
+
+@pytest.mark.gpu
+def test_dosxyz_photon_flight_mode_follows_environment(tmp_path):
+    """OMC_PRECISION=fp32 gives delta-tracked photon flights unless
+    OMC_WOODCOCK=0 asks for the reference's voxel stepping; FP64 steps
+    voxel by voxel unless OMC_WOODCOCK=1."""
+    import os
+    stem = write_case("water2MeV", tmp_path, ncase=2000)
+    for env, expect in (({"OMC_PRECISION": "fp32"}, "delta tracking"),
+                        ({"OMC_PRECISION": "fp32", "OMC_WOODCOCK": "0"}, "voxel stepping"),
+                        ({}, "voxel stepping"),
+                        ({"OMC_WOODCOCK": "on"}, "delta tracking")):
+        e = {k: v for k, v in os.environ.items() if k not in ("OMC_PRECISION", "OMC_WOODCOCK")}
+        e.update(env)
+        r = subprocess.run([str(BIN / "omc_dosxyz"), "-i", str(stem), "-o", "out"], capture_output=True, text=True,
+                           timeout=300, env=e, cwd=tmp_path)
+        assert r.returncode == 0, r.stderr
+        assert f"photon flights by {expect}" in r.stdout, (env, r.stdout[-500:])
