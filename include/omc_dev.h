@@ -100,7 +100,8 @@ typedef struct {
     double  host_ms;            /* scheduler thread busy (admission, launches,
                                    generation ends), not waiting */
     double  pipe_idle_ms;       /* summed over pipelines: device time between a
-                                   generation's end and the next one's start */
+                                   generation's end and the next one's start,
+                                   within the run */
     int64_t woodcock_over;      /* delta-tracking sites whose attenuation
                                    exceeded the majorant (0 unless the
                                    majorant is wrong) */

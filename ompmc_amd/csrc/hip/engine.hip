@@ -132,6 +132,7 @@ struct Pipe {
     hipEvent_t ev_el0 = nullptr, ev_el1 = nullptr, ev_begin = nullptr;
     hipEvent_t ev_end[2] = { nullptr, nullptr };   // end of generation gen (slot gen & 1)
     uint64_t gen = 0;             // generations launched on this pipeline
+    uint64_t idle_from = 0;       // first generation of the current run (idle counts after it)
     int cur = 0;
     uint32_t nP = 0, nE = 0;      // particles waiting in the current queues
     bool inflight = false;
@@ -1005,8 +1006,10 @@ static int complete_generation(omc_dev_ctx *c, int pi)
     P.nE = std::min(P.h->counts[2 + P.cur], c->qcap);
     omc_dev_stats &S = c->acc;
     float ms = 0.f;
-    if (P.gen > 0) {
-        // stream idle between the previous generation's end and this one's start
+    if (P.gen > P.idle_from) {
+        // stream idle between the previous generation's end and this one's
+        // start (not before a run's first generation: that gap is the
+        // caller's time between runs)
         HIPCHK(hipEventElapsedTime(&ms, P.ev_end[(P.gen - 1) & 1], P.ev_begin));
         S.pipe_idle_ms += ms;
     }
@@ -1149,7 +1152,10 @@ static int turn_any(omc_dev_ctx *c)
 static void stats_begin(omc_dev_ctx *c)
 {
     memset(&c->acc, 0, sizeof c->acc);
-    for (auto &P : c->pipes) P.base = P.h->status;
+    for (auto &P : c->pipes) {
+        P.base = P.h->status;
+        P.idle_from = P.gen;
+    }
 }
 
 static void stats_end(omc_dev_ctx *c, omc_dev_stats *out, int64_t histories, double t_start)
