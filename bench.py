@@ -203,7 +203,7 @@ def roofline(stats, precision):
         "frac": achieved / HBM_PEAK_GBS, "traffic": ratio * alg_launch if ratio else None,
         "traffic_over_algorithmic": ratio,
         "traffic_source": ({"file": f"profiles/{name}_slab6mv_fp32_traffic.md",
-                            "profiled_run": "this slab workload at the same size (2e7 histories per step)",
+                            "profiled_run": "this slab workload at the same size (1e8 histories per step)",
                             "measured_bytes_per_launch": measured,
                             "algorithmic_bytes_per_launch": alg_profiled,
                             "here": "measured/algorithmic of the profiled run x this run's algorithmic bytes per launch"}
@@ -414,7 +414,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--hist", type=int, default=20_000_000)
+    ap.add_argument("--hist", type=int, default=100_000_000)
     ap.add_argument("--pipelines", type=int, default=0)
     ap.add_argument("--chunk", type=int, default=0)
     ap.add_argument("--cpu-sample", type=int, default=10_000_000)
@@ -468,7 +468,7 @@ def main():
     out["device_stats_fp32_last_step"] = st32[-1]
     if not args.no_tune:
         log("autotune FP32 on the slab workload")
-        out["autotune"] = autotune_section(prob, D.FP32, probe=args.hist)
+        out["autotune"] = autotune_section(prob, D.FP32, probe=min(args.hist, 20_000_000))
     if not args.no_beamlets or not args.no_latency:
         pprob, src, bl_nh = patient_problem(work)
         if not args.no_beamlets:
