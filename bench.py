@@ -5,36 +5,47 @@ value: histories/s of the FP32-transport engine (FP64 tallies) on the
 heterogeneous slab phantom -- 64^3 voxels at 0.25 cm, water 0-3 cm / lung
 (0.26) 3-9 cm / bone (1.85) 9-11 cm / water to 16 cm -- irradiated by the
 synthetic 6 MV spectrum, 10x10 cm^2 field at SSD 100, ECUT 0.7 MeV, PCUT
-0.01 MeV, 10 batches.  One step = one device run of --hist histories (source,
+0.01 MeV, 50 batches.  One step = one device run of --hist histories (source,
 transport, batch fold); tables and phantom are resident before the timed
-region.  The plain FP64 kernel runs the same steps beside it.
+region.  Before the timed steps the autotuner picks the engine settings on
+the same engine (a probe of the same workload); the timed steps run with its
+pick.  The plain FP64 kernel runs the same steps beside it.
 
 Also reported:
-  roofline     the dominant transport kernel's algorithmic HBM bytes (16 B per
-               FP64 tally update, 1 B medium + 4/8 B density per voxel
-               entered, counted on the device) over its launch durations
-               (HIP events on the launching stream)
-  beamlets     matRad-style dose influence on the 128x128x96 patient phantom:
-               400 beamlets of 0.5x0.5 cm^2 at gantry 0, histories
+  roofline     algorithmic HBM bytes per step (16 B per FP64 tally update,
+               1 B medium + 4/8 B density per voxel entered, counted on the
+               device) over the step time: a device-level fraction of the
+               8 TB/s peak; per_launch keeps the dominant kernel's figure per
+               launch (HIP events on its stream; the pipelines overlap)
+  beamlets     C3 matRad-style dose influence on the 128x128x96 patient
+               phantom: 400 beamlets of 0.5x0.5 cm^2 at gantry 0, histories
                log-uniform in [1e4, 1e6], all in one device pass with the
-               sparse Dij (1% threshold) assembled on the device; FP32 and FP64,
-               and the plain FP64 method (one beamlet per job, in turn)
-  latency      Poisson stream of single-beamlet requests, each sized for 2%
-               relative sigma at its Dmax voxel, served by continuous batching;
-               p50/p99 of submit -> result, FP32 and FP64 at the same load
-  autotune     what omc_dev_tune picks for the FP32 engine on the slab
-               workload (probe = one step's histories) and the rate it measured
+               sparse Dij (1% threshold) assembled on the device; FP32 and FP64
+               batched, the plain FP64 method (one beamlet per job, in turn),
+               and the equal-accuracy gate FP32 vs FP64
+  latency      single-beamlet requests served by continuous batching, each
+               run until its relative sigma at Dmax is <= 2% (sized from a
+               pilot table, extended by whole batches when it is not):
+               p50/p99 of first submit -> final result, FP32 and FP64
+  c4           the serving stream: three resident phantoms, Poisson arrivals,
+               square fields of 1-20 cm, mono 1-6 MeV or 6 MV, histories
+               log-uniform in [1e5, 1e8]; p50/p99 and aggregate histories/s
+  c5           the large multi-beam plan: 256x256x160 at 0.2 cm, five 8x8 cm^2
+               6 MV beams at gantry 0/72/144/216/288, 1e9 histories, FP32 vs
+               FP64 with the equal-accuracy gate
   cpu_baseline the reference's OpenMP omc_dosxyz on a bounded sample of the
-               slab workload, on this host's cores
+               slab workload, on this host's CPU share
 
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--hist H]
 """
 from __future__ import annotations
 
 import argparse
+import ctypes as C
 import json
 import math
 import os
+import platform
 import re
 import subprocess
 import sys
@@ -55,6 +66,7 @@ if int(os.environ.get("GPU_MAX_HW_QUEUES", "0") or 0) < 8:
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "histories/s and p50/p99 beamlet-dose latency at 2% σ, FP32-batched vs FP64 GPU"
+TUNE_KEYS = ("pipelines", "chunk", "yield_trips", "drain_lanes", "drain_trips", "event_div", "fetch_min")
 
 
 def log(msg):
@@ -67,35 +79,82 @@ def slab_problem(workdir: Path, ncase: int):
     ph = cfg.slab_phantom()
     pf = ph.write_egsphant(workdir / "slab.egsphant")
     spec = cfg.write_spectrum(workdir / "6mv.spectrum")
-    items = cfg.dosxyz_inp_items(workdir, pf, ncase=ncase, nbatch=10, spectrum_file=spec,
+    items = cfg.dosxyz_inp_items(workdir, pf, ncase=ncase, nbatch=50, spectrum_file=spec,
                                  field=(-5.0, 5.0, -5.0, 5.0), ssd=100.0)
     cfg.write_inp(workdir / "c2", items)
     return P.load_problem(workdir / "c2"), workdir / "c2"
 
 
-def patient_problem(workdir: Path):
-    """128x128x96 patient phantom at 0.25 cm with the 6 MV spectrum and its
-    400 beamlets at gantry 0 (isocentre = phantom centre, SAD 100)."""
+def patient_problem(workdir: Path, spec_name="c3", phantom=None, beams=None):
+    """A patient-like phantom (default: the 128x128x96 C3 one at 0.25 cm)
+    with the 6 MV spectrum and a matRad beamlet source over it (default: the
+    400 C3 beamlets at gantry 0, isocentre = phantom centre, SAD 100)."""
     from ompmc_amd import _native as N, configs as cfg, problem as P
-    ph = cfg.patient_phantom()
-    pf = ph.write_egsphant(workdir / "patient.egsphant")
+    ph = phantom if phantom is not None else cfg.patient_phantom()
+    pf = ph.write_egsphant(workdir / f"{spec_name}.egsphant")
     spec = cfg.write_spectrum(workdir / "6mv.spectrum")
     items = cfg.dosxyz_inp_items(workdir, pf, ncase=1000, nbatch=10, spectrum_file=spec)
-    cfg.write_inp(workdir / "c3", items)
-    prob = P.load_problem(workdir / "c3")
-    bs = cfg.beamlets(gantry_deg=(0.0,), field=10.0, bixel=0.5, sad=100.0)
+    cfg.write_inp(workdir / spec_name, items)
+    prob = P.load_problem(workdir / spec_name)
+    bs = beams if beams is not None else cfg.beamlets(gantry_deg=(0.0,), field=10.0, bixel=0.5, sad=100.0)
     src = N.beamlet_source(prob.src.spec, bs.ibeam, bs.src, bs.corner, bs.side1, bs.side2)
-    nhist = cfg.loguniform_histories(len(bs.ibeam), 1e4, 1e6, seed=2024)
-    return prob, src, nhist
+    return prob, src
+
+
+def density(prob):
+    n = prob.ph.isize * prob.ph.jsize * prob.ph.ksize
+    return np.ctypeslib.as_array(prob.ph.rho, shape=(n,)).copy()
+
+
+# ------------------------------------------------------------ statistics --
+def equal_accuracy(a, b, rho=None):
+    """Run `a` (FP32) against run `b` (FP64), independent streams, each as
+    (sum, sum of squares, batches) per region or (mean, variance of the
+    mean, degrees of freedom) when given as ("mv", m, v, dof): the fraction
+    of voxels above 10% of the maximum (selected on the mean of both, so
+    neither run's fluctuations choose the voxels) whose means agree within
+    two combined standard errors, z statistics, and the integral ratio."""
+    from scipy import stats
+
+    def mv(r):
+        if isinstance(r[0], str):
+            return r[1], r[2], r[3]
+        s, s2, nb = r
+        m = s[1:] / nb
+        return m, np.maximum(s2[1:] / nb - m * m, 0.0) / (nb - 1), nb - 1
+    ma, va, da = mv(a)
+    mb, vb, db = mv(b)
+    dose = 0.5 * (ma + mb)
+    if rho is not None:
+        dose = np.where(rho < 0.044, 0.0, dose / np.maximum(rho, 1e-30))
+    sel = dose > 0.1 * dose.max()
+    n = int(sel.sum())
+    z = (ma - mb)[sel] / np.sqrt(va + vb + 1e-300)[sel]
+    frac = float(np.mean(np.abs(z) <= 2.0))
+    ratio = float(ma.sum() / mb.sum())
+    dof = int(da + db)
+    expect = float(2.0 * stats.t.cdf(2.0, dof) - 1.0)
+    return {"voxels_above_10pct_dmax": n, "fraction_within_2sigma": frac, "expected_if_identical": expect,
+            "z_mean": float(z.mean()), "z_rms": float(np.sqrt(np.mean(z * z))), "max_abs_z": float(np.abs(z).max()),
+            "integral_energy_fp32_over_fp64": ratio, "degrees_of_freedom": dof,
+            "gates": "at least 95% of the voxels above 10% of Dmax within 2 combined sigma, "
+                     "integral energy within 0.5%",
+            "pass": bool(frac >= 0.95 and abs(ratio - 1.0) <= 0.005)}
 
 
 # -------------------------------------------------------------- slab run --
-def time_slab(prob, precision, hist, steps, warmup, pipelines=0, chunk=0, seed0=2000):
+def time_slab(prob, precision, hist, steps, warmup, tune=True, seed0=2000, probe=20_000_000):
+    """Warm-up, the autotuner (on the same engine, a probe of the same
+    workload), then `steps` timed runs with its pick."""
     from ompmc_amd import device as D
     nreg = prob.reg.nreg
-    nbatch = 10
+    # 50 batches per step: the equal-accuracy gate on the timed steps then has
+    # >= 148 degrees of freedom at 3 steps, where 2 sigma covers 95.4%
+    nbatch = 50
     nper = hist // nbatch
-    with D.Engine(precision, pipelines=pipelines, chunk=chunk) as eng:
+    probe = min(probe, hist)
+    tuned = None
+    with D.Engine(precision) as eng:
         eng.upload_media(prob.tab)
         pid = eng.upload_phantom(prob.ph, prob.reg)
         # untimed warm-up: the requested steps, and at least 3 s of transport
@@ -104,7 +163,16 @@ def time_slab(prob, precision, hist, steps, warmup, pipelines=0, chunk=0, seed0=
         while w < warmup or time.perf_counter() - t_w < 3.0:
             eng.run_dosxyz(pid, prob.src, nper, nbatch, seed=(1000 + w, 7), nreg=nreg)
             w += 1
-        stats, etot, sums = [], [], []
+        if tune:
+            t0 = time.perf_counter()
+            eng.run_dosxyz(pid, prob.src, probe // nbatch, nbatch, seed=(4, 4), nreg=nreg)
+            default_hps = probe / (time.perf_counter() - t0)
+            t0 = time.perf_counter()
+            best, hps = eng.tune(pid, prob.src, probe, nbatch=nbatch, seed=(4, 4))
+            tuned = {"probe_histories": probe, "seconds": time.perf_counter() - t0,
+                     "defaults_histories_per_s": default_hps, "tuned_histories_per_s": hps,
+                     "tuned_over_defaults": hps / default_hps, "chosen": {k: best[k] for k in TUNE_KEYS}}
+        stats, sums = [], []
         t0 = time.perf_counter()
         for k in range(steps):
             s, s2, st, _ = eng.run_dosxyz(pid, prob.src, nper, nbatch, seed=(seed0 + k, 7), nreg=nreg)
@@ -114,56 +182,7 @@ def time_slab(prob, precision, hist, steps, warmup, pipelines=0, chunk=0, seed0=
     etot = [float(s[1:].sum()) for s, _ in sums]
     # the steps' batches together: steps x nbatch batches of nper histories
     batches = (sum(s for s, _ in sums), sum(s2 for _, s2 in sums), steps * nbatch)
-    return el, stats, etot, nper * nbatch, batches
-
-
-def accuracy_gates(prob, a, b):
-    """FP32 run `a` against FP64 run `b` (independent seeds), each as
-    (sum, sum of squares, batches) per region: the fraction of voxels above
-    10% of the FP64 Dmax whose batch means agree within two combined
-    standard errors, and the ratio of the integral energies."""
-    rho = density(prob)
-
-    def mean_var(s, s2, nb):
-        m = s[1:] / nb
-        return m, np.maximum(s2[1:] / nb - m * m, 0.0) / (nb - 1)
-
-    from scipy import stats
-    ma, va = mean_var(*a)
-    mb, vb = mean_var(*b)
-    dose_b = np.where(rho < 0.044, 0.0, mb / np.maximum(rho, 1e-30))
-    sel = dose_b > 0.1 * dose_b.max()
-    n = int(sel.sum())
-    frac = float(np.mean(np.abs(ma - mb)[sel] <= 2.0 * np.sqrt(va + vb)[sel]))
-    ratio = float(a[0][1:].sum() / b[0][1:].sum())
-    # sigma is estimated from the batches, so even two identical engines put
-    # only P(|t| <= 2) of the voxels within 2 sigma (94.98% at 58 degrees of
-    # freedom); below 95% the gate allows three binomial standard deviations
-    # under that expectation
-    expect = float(2.0 * stats.t.cdf(2.0, a[2] + b[2] - 2) - 1.0)
-    floor = min(0.95, expect - 3.0 * math.sqrt(expect * (1.0 - expect) / max(n, 1)))
-    return {"voxels_above_10pct_dmax": n, "fraction_within_2sigma": frac,
-            "expected_if_identical": expect, "integral_energy_fp32_over_fp64": ratio,
-            "batches": [a[2], b[2]],
-            "gates": "at least 95% of those voxels within 2 sigma (or within 3 binomial sd of the coverage "
-                     "expected with batch-estimated sigma, if that is below 95%), integral energy within 0.5%",
-            "pass": bool(frac >= floor and abs(ratio - 1.0) <= 0.005)}
-
-
-def autotune_section(prob, precision, probe=4_000_000):
-    """omc_dev_tune on this workload: the settings it picks (from the engine
-    defaults) and the histories/s it measured for them on the probe."""
-    from ompmc_amd import device as D
-    with D.Engine(precision) as eng:
-        eng.upload_media(prob.tab)
-        pid = eng.upload_phantom(prob.ph, prob.reg)
-        eng.run_dosxyz(pid, prob.src, 100000, 10, seed=(3, 3), nreg=prob.reg.nreg)      # warm-up
-        t0 = time.perf_counter()
-        best, hps = eng.tune(pid, prob.src, probe, nbatch=10, seed=(4, 4))
-        el = time.perf_counter() - t0
-    keys = ("pipelines", "chunk", "yield_trips", "drain_lanes", "drain_trips", "drain_min", "event_div", "fetch_min")
-    return {"probe_histories": probe, "seconds": el, "histories_per_s": hps,
-            "chosen": {k: best[k] for k in keys}}
+    return el, stats, etot, nper * nbatch, batches, tuned
 
 
 def profiled_traffic(kernel):
@@ -179,61 +198,111 @@ def profiled_traffic(kernel):
     return (float(t.group(1)), float(a.group(1))) if t and a else (None, None)
 
 
-def roofline(stats, precision):
-    """Dominant kernel of the transport pass and its achieved algorithmic
-    bandwidth: bytes = 16 N_dep + (9 | 5) N_cross of the particles it ran."""
+def profiled_valu(kernel):
+    """VALU busy and lanes active per VALU instruction of `kernel` from the
+    committed counter summary (rocprofv3 --pmc, the kernel run alone)."""
+    f = ROOT / "profiles" / "k_electron_k_photon_slab6mv_fp32_counters.md"
+    if not f.exists():
+        return None
+    text = f.read_text()
+    m = re.search(rf"{kernel}_valu: busy ([0-9.]+) utilization ([0-9.]+) lanes ([0-9.]+)", text)
+    if not m:
+        return None
+    busy, util, lanes = (float(x) for x in m.groups())
+    return {"source": f"profiles/{f.name}", "valu_busy": busy, "valu_utilization": util,
+            "active_lanes_per_valu_instruction": lanes, "effective_lane_throughput": busy * util,
+            "note": "VALU issue is the bound of this latency-bound, divergent transport; HBM is not"}
+
+
+def roofline(stats, precision, ms_per_step, steps, n_units_oracle=None):
+    """Device-level roofline of the transport over the timed steps.
+
+    achieved = algorithmic bytes per step / step time, where the algorithmic
+    bytes are 16 B per FP64 tally update plus (1 + 8|4) B per voxel entered
+    (medium + density), counted by the device; frac = achieved / 8 TB/s.
+    per_launch keeps the dominant kernel's own figure (its algorithmic bytes
+    over its average launch duration, HIP events on its stream); the six
+    pipelines' launches overlap, so per-launch figures do not add up to the
+    device's."""
     per_cross = 9 if precision == 0 else 5
     tot = {k: sum(s[k] for s in stats) for k in stats[0]}
+    hist = tot["histories"]
+    step_bytes = (16 * tot["deposits"] + per_cross * (tot["photon_crossings"] + tot["electron_crossings"])) / steps
+    achieved = step_bytes / (ms_per_step / 1e3) / 1e9
     kern = {}
     for name, pre in (("k_photon", "photon"), ("k_electron", "electron")):
         ms = tot[f"{pre}_kernel_ms"]
         launches = tot[f"{pre}_launches"]
         byts = 16 * tot[f"{pre}_deposits"] + per_cross * tot[f"{pre}_crossings"]
         kern[name] = dict(ms=ms, launches=launches, bytes=byts)
-    name = max(kern, key=lambda k: kern[k]["ms"])
-    k = kern[name]
-    achieved = k["bytes"] / (k["ms"] / 1e3) / 1e9 if k["ms"] > 0 else 0.0
-    hist = tot["histories"]
-    measured, alg_profiled = profiled_traffic(name) if precision == 1 else (None, None)
-    alg_launch = k["bytes"] / max(k["launches"], 1)
-    # the profiled run's measured/algorithmic ratio applied to this run's launches
+    dom = max(kern, key=lambda k: kern[k]["ms"])
+    k = kern[dom]
+    measured, alg_profiled = profiled_traffic(dom) if precision == 1 else (None, None)
     ratio = measured / alg_profiled if measured else None
-    return {
+    out = {
         "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-        "frac": achieved / HBM_PEAK_GBS, "traffic": ratio * alg_launch if ratio else None,
+        "frac": achieved / HBM_PEAK_GBS,
+        "traffic": ratio * step_bytes if ratio else None,
+        "level": "device: algorithmic bytes per step / ms_per_step",
+        "algorithmic_bytes_per_step": step_bytes,
         "traffic_over_algorithmic": ratio,
-        "traffic_source": ({"file": f"profiles/{name}_slab6mv_fp32_traffic.md",
-                            "profiled_run": "this slab workload at the same size (1e8 histories per step)",
+        "traffic_source": ({"file": f"profiles/{dom}_slab6mv_fp32_traffic.md",
+                            "profiled_run": "this slab workload, FP32, 1e8 histories per step",
                             "measured_bytes_per_launch": measured,
                             "algorithmic_bytes_per_launch": alg_profiled,
-                            "here": "measured/algorithmic of the profiled run x this run's algorithmic bytes per launch"}
-                           if ratio else None),
-        "kernel": name, "launches": k["launches"],
-        "algorithmic_bytes_per_launch": alg_launch,
-        "avg_launch_ms": k["ms"] / max(k["launches"], 1),
+                            "here": "measured/algorithmic of the profiled run x this run's algorithmic bytes "
+                                    "per step"} if ratio else None),
         "per_history": {"deposits": tot["deposits"] / hist,
                         "photon_crossings": tot["photon_crossings"] / hist,
                         "electron_crossings": tot["electron_crossings"] / hist,
-                        "bytes": (16 * tot["deposits"] + per_cross *
-                                  (tot["photon_crossings"] + tot["electron_crossings"])) / hist},
-        "all_kernels_GBs": {n: (v["bytes"] / (v["ms"] / 1e3) / 1e9 if v["ms"] else 0.0) for n, v in kern.items()},
+                        "bytes": step_bytes * steps / hist},
+        "per_launch": {"kernel": dom, "launches": k["launches"],
+                       "algorithmic_bytes_per_launch": k["bytes"] / max(k["launches"], 1),
+                       "avg_launch_ms": k["ms"] / max(k["launches"], 1),
+                       "achieved_GBs": k["bytes"] / (k["ms"] / 1e3) / 1e9 if k["ms"] else 0.0,
+                       "all_kernels_GBs": {n: (v["bytes"] / (v["ms"] / 1e3) / 1e9 if v["ms"] else 0.0)
+                                           for n, v in kern.items()}},
     }
+    if n_units_oracle:
+        # the same byte model on the reference's own units: its deposits and
+        # its voxel crossings per history (voxel-stepped photons), as the
+        # FP64 replay engine counts them on this workload
+        b = 16 * n_units_oracle["deposits"] + per_cross * n_units_oracle["crossings"]
+        hps = hist / steps / (ms_per_step / 1e3)
+        out["oracle_units"] = {"bytes_per_history": b, "deposits_per_history": n_units_oracle["deposits"],
+                               "crossings_per_history": n_units_oracle["crossings"],
+                               "achieved": hps * b / 1e9, "frac": hps * b / 1e9 / HBM_PEAK_GBS}
+    v = profiled_valu(dom) if precision == 1 else None
+    if v:
+        out["valu"] = v
+    return out
 
 
 # --------------------------------------------------------- beamlet batch --
-def beamlet_batch(prob, src, nhist, precision, nbatch=10, engine=None):
+def beamlet_batch(prob, src, nhist, precision, nbatch=10, engine=None, tune=False):
     from ompmc_amd import device as D
     with D.Engine(precision, **(engine or {})) as eng:
         eng.upload_media(prob.tab)
         pid = eng.upload_phantom(prob.ph, prob.reg)
         small = np.minimum(nhist, 2000)
         eng.beamlets_sparse(pid, src, small, nbatch, 0.01, seed=(5, 5))          # warm-up
+        tuned = None
+        if tune:
+            # probe: every beamlet at a fifth of its histories
+            probe = np.maximum(nhist // 5, nbatch)
+            t0 = time.perf_counter()
+            best, hps = eng.tune_beamlets(pid, src, probe, nbatch=nbatch, seed=(7, 7))
+            tuned = {"probe_histories": int(probe.sum()), "seconds": time.perf_counter() - t0,
+                     "tuned_histories_per_s": hps, "chosen": {k: best[k] for k in TUNE_KEYS}}
         t0 = time.perf_counter()
         jcs, irs, vals, st = eng.beamlets_sparse(pid, src, nhist, nbatch, 0.01, seed=(6, 6))
         el = time.perf_counter() - t0
     h = int(st.histories)
-    return {"seconds": el, "histories": h, "histories_per_s": h / el, "nnz": int(len(irs)),
-            "device_transport_ms": st.transport_ms}, (jcs, irs, vals)
+    out = {"seconds": el, "histories": h, "histories_per_s": h / el, "nnz": int(len(irs)),
+           "device_transport_ms": st.transport_ms}
+    if tuned:
+        out["autotune"] = tuned
+    return out
 
 
 def beamlet_plain(prob, src, nhist, precision, nbatch=10):
@@ -256,6 +325,43 @@ def beamlet_plain(prob, src, nhist, precision, nbatch=10):
         el = time.perf_counter() - t0
     return {"seconds": el, "histories": total, "histories_per_s": total / el,
             "method": "one beamlet per job, run to completion in turn"}
+
+
+def beamlet_gate(prob, src, nhist, nbatch=100, group=40):
+    """Equal-accuracy gate of the C3 beamlets, FP32 (engine default) against
+    FP64, independent streams: the summed dose of all beamlets (the 10x10 cm
+    field), whose variance is the sum of the beamlets' batch variances (the
+    beamlets are independent; each has nbatch - 1 degrees of freedom, the
+    sum at least as many).  100 batches per beamlet, so that the t quantile
+    of 2 sigma covers 95% (with 10 it covers 92%).  Beamlets run in groups of
+    `group` so that their batch sums fit host memory."""
+    from ompmc_amd import device as D
+    nreg = prob.reg.nreg
+    res = {}
+    for name, prec, seed in (("fp32", D.FP32, (61, 1)), ("fp64", D.FP64, (62, 1))):
+        m = np.zeros(nreg - 1)
+        v = np.zeros(nreg - 1)
+        with D.Engine(prec) as eng:
+            eng.upload_media(prob.tab)
+            pid = eng.upload_phantom(prob.ph, prob.reg)
+            from ompmc_amd import _native as N
+            for g0 in range(0, src.nbeamlets, group):
+                idx = np.arange(g0, min(g0 + group, src.nbeamlets))
+                sub = N.beamlet_source(src.spec, np.zeros(len(idx), np.int32), src_array(src, "src")[:3],
+                                       src_array(src, "corner").reshape(-1, 3)[idx],
+                                       src_array(src, "side1").reshape(-1, 3)[idx],
+                                       src_array(src, "side2").reshape(-1, 3)[idx])
+                s, s2, _ = eng.run_beamlets(pid, sub, nhist[idx], nbatch, seed=(seed[0], seed[1] + g0))
+                mb = s[:, 1:] / nbatch
+                m += mb.sum(0)
+                v += (np.maximum(s2[:, 1:] / nbatch - mb * mb, 0.0) / (nbatch - 1)).sum(0)
+        res[name] = (m, v)
+    (m32, v32), (m64, v64) = res["fp32"], res["fp64"]
+    return equal_accuracy(("mv", m32, v32, nbatch - 1), ("mv", m64, v64, nbatch - 1), rho=density(prob))
+
+
+def src_array(src, name):
+    return src._keep[name]
 
 
 # --------------------------------------------------------------- latency --
@@ -287,107 +393,298 @@ def sigma_at_dmax(s, s2, nbatch, rho):
     return math.sqrt(v[i]) / m[i] if m[i] > 0 else float("inf")
 
 
-def density(prob):
-    n = prob.ph.isize * prob.ph.jsize * prob.ph.ksize
-    return np.ctypeslib.as_array(prob.ph.rho, shape=(n,)).copy()
+class SigmaRequest:
+    """A dose request run until its relative sigma at Dmax is <= target: it
+    starts with `nperbatch` x `nbatch` histories; when a part ends above the
+    target it is extended by whole batches of the same size (independent
+    streams), enough for the target by the 1/sqrt(N) law plus a margin, and
+    the batches are pooled."""
+
+    def __init__(self, eng, pid, src, beamlet, nperbatch, nbatch, seed, rho, nreg, target=0.02):
+        self.eng, self.pid, self.src, self.beamlet = eng, pid, src, beamlet
+        self.nper, self.rho, self.nreg, self.target = nperbatch, rho, nreg, target
+        self.seed = seed
+        self.s = np.zeros(nreg)
+        self.s2 = np.zeros(nreg)
+        self.nb = 0
+        self.parts = 0
+        self.first_submit = None
+        self.sigma = float("inf")
+        self.histories = 0
+        self._submit(nbatch)
+
+    def _submit(self, nbatch):
+        self.ticket = self.eng.submit(self.pid, self.src, self.nper * nbatch, nbatch=nbatch,
+                                      seed=(self.seed[0], self.seed[1] + 7919 * self.parts), kind=1,
+                                      beamlet=self.beamlet)
+        self.part_nb = nbatch
+        self.parts += 1
+
+    def poll(self):
+        """None while running; the latency in ms once the target is met."""
+        res, s, s2 = self.eng.poll(self.ticket, self.nreg)
+        if not res.done:
+            return None
+        if res.status != 0:
+            raise RuntimeError(f"request failed: {res.status}")
+        if self.first_submit is None:             # parts end in the order they were submitted
+            self.first_submit = res.submit_ms
+        self.s += s
+        self.s2 += s2
+        self.nb += self.part_nb
+        self.histories += res.nhist
+        self.sigma = sigma_at_dmax(self.s, self.s2, self.nb, self.rho)
+        if self.sigma <= self.target:
+            return res.finish_ms - self.first_submit
+        need = int(math.ceil(self.nb * ((self.sigma / self.target) ** 2 * 1.15 - 1.0)))
+        self._submit(max(need, 2))
+        return None
 
 
-def serve_stream(prob, src, beamlets, nh, precision, rate, arrivals, nbatch=10, engine=None):
+def latency_section(prob, src, nreq=48, load=0.7, seed=77, nbatch=10, target=0.02):
+    """Single-beamlet requests run to sigma <= 2% at Dmax.  Each beamlet's
+    first size comes from a pilot table (all 400 beamlets at 2e4 histories in
+    one untimed pass, sigma scaled by 1/sqrt(N) with a 25% margin); a request
+    that ends above 2% is extended (SigmaRequest).  Poisson arrivals at
+    `load` of the FP64 engine's closed-loop capacity on the same requests;
+    the same schedule for both precisions."""
     from ompmc_amd import device as D
     nreg = prob.reg.nreg
     rho = density(prob)
-    lat, done = [], []
-    with D.Engine(precision, **(engine or {})) as eng:
-        eng.upload_media(prob.tab)
-        pid = eng.upload_phantom(prob.ph, prob.reg)
-        # warm-up request
-        t = eng.submit(pid, src, 20000, nbatch=nbatch, seed=(1, 1), kind=1, beamlet=int(beamlets[0]))
-        wait_ticket(eng, t, nreg)
-        open_t = {}
-        i = 0
-        t0 = time.perf_counter()
-        total = 0
-        while i < len(arrivals) or open_t:
-            now = time.perf_counter() - t0
-            while i < len(arrivals) and arrivals[i] <= now:
-                tk = eng.submit(pid, src, nh, nbatch=nbatch, seed=(100 + i, 3), kind=1, beamlet=int(beamlets[i]))
-                open_t[tk] = i
-                i += 1
-            left = eng.step() if open_t else 0
-            for tk in list(open_t):
-                res, s, s2 = eng.poll(tk, nreg)
-                if res.done:
-                    if res.status != 0:
-                        raise RuntimeError(f"request {tk} failed: {res.status}")
-                    lat.append(res.finish_ms - res.submit_ms)
-                    done.append((s, s2))      # analysed after the stream: the loop only drives the engine
-                    total += res.nhist
-                    del open_t[tk]
-            if not left and not open_t and i < len(arrivals):
-                time.sleep(max(0.0, min(0.002, arrivals[i] - (time.perf_counter() - t0))))
-        el = time.perf_counter() - t0
-    sig = [sigma_at_dmax(s, s2, nbatch, rho) for s, s2 in done]
-    lat = np.array(lat)
-    if os.environ.get("OMC_BENCH_LATENCIES"):
-        log(f"latencies ms ({'fp32' if precision else 'fp64'}): " + " ".join(f"{v:.0f}" for v in lat))
-    return {"requests": len(lat), "histories_per_request": nh, "offered_rate_per_s": rate,
-            "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
-            "mean_ms": float(lat.mean()), "histories_per_s": total / el,
-            "sigma_at_dmax_median": float(np.median(sig)), "sigma_at_dmax_max": float(np.max(sig))}
-
-
-def latency_section(prob, src, nreq=48, load=0.7, seed=77, engine=None):
-    """Requests sized for 2% sigma at Dmax; Poisson arrivals at `load` of the
-    FP64 engine's closed-loop capacity, the same schedule for both precisions."""
-    from ompmc_amd import device as D
-    nreg = prob.reg.nreg
     rng = np.random.default_rng(seed)
-    centre = src.nbeamlets // 2 + int(math.sqrt(src.nbeamlets)) // 2
-    # pilot: sigma at Dmax of the central beamlet
+    beamlets = rng.integers(0, src.nbeamlets, size=nreq)
+    npil = 20000
+    sig0 = np.zeros(src.nbeamlets)
     with D.Engine(D.FP32) as eng:
         eng.upload_media(prob.tab)
         pid = eng.upload_phantom(prob.ph, prob.reg)
-        npil = 50000
-        t = eng.submit(pid, src, npil, nbatch=10, seed=(9, 9), kind=1, beamlet=centre)
-        _, s, s2 = wait_ticket(eng, t, nreg)
-    sig0 = sigma_at_dmax(s, s2, 10, density(prob))
-    nh = int(min(max(npil * (sig0 / 0.02) ** 2, 1e4), 5e6))
-    nh -= nh % 10
-    beamlets = rng.integers(0, src.nbeamlets, size=nreq)
-    # capacity of the FP64 engine on these requests (closed loop)
-    with D.Engine(D.FP64, **(engine or {})) as eng:
+        for g0 in range(0, src.nbeamlets, 40):
+            tk = [eng.submit(pid, src, npil, nbatch=nbatch, seed=(9, 9 + b), kind=1, beamlet=b)
+                  for b in range(g0, min(g0 + 40, src.nbeamlets))]
+            for b, t in zip(range(g0, g0 + len(tk)), tk):
+                _, s, s2 = wait_ticket(eng, t, nreg)
+                sig0[b] = sigma_at_dmax(s, s2, nbatch, rho)
+    nper = np.maximum((npil * (sig0 / target) ** 2 * 1.25 / nbatch).astype(np.int64), 100)
+
+    def stream(prec, arrivals):
+        lat, sig, hist, parts = [], [], 0, []
+        with D.Engine(prec) as eng:
+            eng.upload_media(prob.tab)
+            pid = eng.upload_phantom(prob.ph, prob.reg)
+            wait_ticket(eng, eng.submit(pid, src, 20000, nbatch=nbatch, seed=(1, 1), kind=1, beamlet=0), nreg)
+            open_r, i = [], 0
+            t0 = time.perf_counter()
+            while i < len(arrivals) or open_r:
+                now = time.perf_counter() - t0
+                while i < len(arrivals) and arrivals[i] <= now:
+                    b = int(beamlets[i])
+                    open_r.append(SigmaRequest(eng, pid, src, b, int(nper[b]), nbatch, (100 + i, 3), rho, nreg,
+                                               target))
+                    i += 1
+                left = eng.step() if open_r else 0
+                for r in list(open_r):
+                    ms = r.poll()
+                    if ms is not None:
+                        lat.append(ms)
+                        sig.append(r.sigma)
+                        parts.append(r.parts)
+                        hist += r.histories
+                        open_r.remove(r)
+                if not left and not open_r and i < len(arrivals):
+                    time.sleep(max(0.0, min(0.002, arrivals[i] - (time.perf_counter() - t0))))
+            el = time.perf_counter() - t0
+        lat = np.array(lat)
+        return {"requests": len(lat), "p50_ms": float(np.percentile(lat, 50)),
+                "p99_ms": float(np.percentile(lat, 99)), "mean_ms": float(lat.mean()),
+                "histories_per_s": hist / el, "mean_histories_per_request": hist / len(lat),
+                "sigma_at_dmax_max": float(max(sig)), "sigma_at_dmax_median": float(np.median(sig)),
+                "requests_extended": int(sum(p > 1 for p in parts))}
+
+    # FP64 closed-loop capacity on the first 8 requests
+    with D.Engine(D.FP64) as eng:
         eng.upload_media(prob.tab)
         pid = eng.upload_phantom(prob.ph, prob.reg)
-        ts = [eng.submit(pid, src, nh, nbatch=10, seed=(50 + k, 1), kind=1, beamlet=int(beamlets[k]))
-              for k in range(8)]
         t0 = time.perf_counter()
-        while eng.step() > 0:
-            pass
+        rs = [SigmaRequest(eng, pid, src, int(beamlets[k]), int(nper[beamlets[k]]), nbatch, (50 + k, 1), rho, nreg,
+                           target) for k in range(8)]
+        while rs:
+            eng.step()
+            rs = [r for r in rs if r.poll() is None]
         cap64 = 8 / (time.perf_counter() - t0)
-        for t in ts:
-            eng.poll(t)
     rate = load * cap64
     arrivals = np.cumsum(rng.exponential(1.0 / rate, size=nreq))
-    out = {"pilot_sigma_at_dmax": sig0, "pilot_histories": npil, "fp64_capacity_requests_per_s": cap64,
-           "load_of_fp64_capacity": load}
+    out = {"target_sigma_at_dmax": target, "pilot_histories_per_beamlet": npil,
+           "fp64_capacity_requests_per_s": cap64, "load_of_fp64_capacity": load, "offered_rate_per_s": rate,
+           "sizing": "pilot table per beamlet (1/sqrt(N), 25% margin); a request ending above the target is "
+                     "extended by whole batches until it meets it; latency = first submit -> final result"}
     for name, prec in (("fp32", D.FP32), ("fp64", D.FP64)):
-        log(f"latency stream {name}: {nreq} requests of {nh} histories at {rate:.2f}/s")
-        out[name] = serve_stream(prob, src, beamlets, nh, prec, rate, arrivals, engine=engine)
+        log(f"latency stream {name}: {nreq} requests at {rate:.2f}/s")
+        out[name] = stream(prec, arrivals)
+    return out
+
+
+# ------------------------------------------------------------ C4 serving --
+def c4_section(workdir, nreq=40, load=0.8, seed=4242):
+    """The serving stream on three resident phantoms (water 40^3 at 0.5 cm,
+    the slab, the patient phantom), one table set for all (their media
+    mapped onto AIR/LUNG/WATER/BONE)."""
+    from ompmc_amd import _native as N, configs as cfg, device as D, problem as P
+    spec = cfg.write_spectrum(workdir / "6mv.spectrum")
+    probs = []
+    for k, ph in enumerate(cfg.c4_phantoms()):
+        pf = ph.write_egsphant(workdir / f"c4_{k}.egsphant")
+        cfg.write_inp(workdir / f"c4_{k}", cfg.dosxyz_inp_items(workdir, pf, ncase=1000, spectrum_file=spec))
+        probs.append(P.load_problem(workdir / f"c4_{k}"))
+    six = probs[0].src.spec
+
+    def source(req):
+        p = probs[req.phantom]
+        src = N.SourceDosxyz()
+        h = req.side_cm / 2
+        N.check(N.host().omc_source_dosxyz_setup(C.byref(src), C.byref(p.ph), -h, h, -h, h, 100.0, 0))
+        if req.energy > 0:
+            N.host().omc_spectrum_mono(C.byref(src.spec), req.energy)
+        else:
+            src.spec = six
+        return src
+
+
+    def run(prec, reqs, rate_note):
+        lat, hist = [], 0
+        with D.Engine(prec) as eng:
+            eng.upload_media(probs[2].tab)
+            pids = [eng.upload_phantom(p.ph, p.reg) for p in probs]
+            wait_done(eng, eng.submit(pids[0], source(reqs[0]), 100000, seed=(1, 1)))
+            srcs = [source(r) for r in reqs]
+            open_t, i = {}, 0
+            t0 = time.perf_counter()
+            while i < len(reqs) or open_t:
+                now = time.perf_counter() - t0
+                while i < len(reqs) and reqs[i].arrival_s <= now:
+                    r = reqs[i]
+                    open_t[eng.submit(pids[r.phantom], srcs[i], r.nhist, nbatch=10, seed=(300 + i, 5))] = i
+                    i += 1
+                left = eng.step() if open_t else 0
+                for tk in list(open_t):
+                    res, _, _ = eng.poll(tk)
+                    if res.done:
+                        if res.status != 0:
+                            raise RuntimeError(f"request {open_t[tk]} failed: {res.status}")
+                        lat.append(res.finish_ms - res.submit_ms)
+                        hist += res.nhist
+                        del open_t[tk]
+                if not left and not open_t and i < len(reqs):
+                    time.sleep(max(0.0, min(0.002, reqs[i].arrival_s - (time.perf_counter() - t0))))
+            el = time.perf_counter() - t0
+        lat = np.array(lat)
+        return {"requests": len(lat), "p50_ms": float(np.percentile(lat, 50)),
+                "p99_ms": float(np.percentile(lat, 99)), "mean_ms": float(lat.mean()),
+                "histories_per_s": hist / el, "seconds": el}
+
+    # capacity: FP64 histories/s on a closed batch of the stream's requests
+    probe = cfg.serving_stream(nreq, 1.0, seed=seed)
+    mean_hist = float(np.mean([r.nhist for r in probe]))
+    with D.Engine(D.FP64) as eng:
+        eng.upload_media(probs[2].tab)
+        pids = [eng.upload_phantom(p.ph, p.reg) for p in probs]
+        reqs = probe[:6]
+        srcs = [source(r) for r in reqs]
+        t0 = time.perf_counter()
+        tks = [eng.submit(pids[r.phantom], s_, r.nhist, seed=(9, k)) for k, (r, s_) in enumerate(zip(reqs, srcs))]
+        while eng.step() > 0:
+            pass
+        cap_hps = sum(r.nhist for r in reqs) / (time.perf_counter() - t0)
+        for t in tks:
+            eng.poll(t)
+    rate = load * cap_hps / mean_hist
+    reqs = cfg.serving_stream(nreq, rate, seed=seed)
+    out = {"requests": nreq, "offered_rate_per_s": rate, "load_of_fp64_capacity": load,
+           "fp64_capacity_histories_per_s": cap_hps, "mean_histories_per_request": mean_hist,
+           "workload": "3 resident phantoms (water 40^3 @ 0.5 cm, slab 64^3 @ 0.25 cm, patient 128x128x96 "
+                       "@ 0.25 cm); Poisson arrivals; square fields U(1,20) cm at SSD 100; mono U(1,6) MeV or "
+                       "6 MV (p = 1/2); histories log-uniform [1e5, 1e8]; nbatch 10"}
+    for name, prec in (("fp32", D.FP32), ("fp64", D.FP64)):
+        log(f"c4 stream {name}: {nreq} requests at {rate:.2f}/s")
+        out[name] = run(prec, reqs, name)
+    for p in probs:
+        P.free_problem(p)
+    return out
+
+
+# ----------------------------------------------------------- C5 plan ------
+def c5_section(workdir, hist=1_000_000_000, nbatch=100):
+    """Five 8x8 cm^2 beams on the 256x256x160 phantom, hist / 5 histories
+    each, FP32 and FP64 (independent streams), timed, and the equal-accuracy
+    gate on the summed dose of the five beams."""
+    from ompmc_amd import configs as cfg, device as D
+    log("c5: building the 256x256x160 phantom")
+    prob, src = patient_problem(workdir, "c5", phantom=cfg.config_phantom("C5"), beams=cfg.c5_beams())
+    nh = np.full(src.nbeamlets, hist // src.nbeamlets, np.int64)
+    rho = density(prob)
+    out = {"workload": "256x256x160 @ 0.2 cm patient-like phantom (C3 geometry scaled, air/lung/water/bone), "
+                       "five 6 MV beams 8x8 cm2 at gantry 0/72/144/216/288, SAD 100, isocentre at the centre",
+           "histories": int(nh.sum()), "nbatch_per_beam": nbatch}
+    res = {}
+    for name, prec, seed in (("fp32", D.FP32, (81, 1)), ("fp64", D.FP64, (82, 1))):
+        log(f"c5 {name}: {int(nh.sum())} histories")
+        with D.Engine(prec) as eng:
+            eng.upload_media(prob.tab)
+            pid = eng.upload_phantom(prob.ph, prob.reg)
+            eng.run_beamlets(pid, src, np.full(src.nbeamlets, 200000, np.int64), nbatch, seed=(1, 1))
+            t0 = time.perf_counter()
+            s, s2, st = eng.run_beamlets(pid, src, nh, nbatch, seed=seed)
+            el = time.perf_counter() - t0
+        mb = s[:, 1:] / nbatch
+        m = mb.sum(0)
+        v = (np.maximum(s2[:, 1:] / nbatch - mb * mb, 0.0) / (nbatch - 1)).sum(0)
+        del s, s2
+        res[name] = (m, v)
+        dose = np.where(rho < 0.044, 0.0, m / np.maximum(rho, 1e-30))
+        i = int(np.argmax(dose))
+        out[name] = {"seconds": el, "histories_per_s": int(nh.sum()) / el, "killed": int(st.killed),
+                     "woodcock_over": int(st.woodcock_over),
+                     "relative_sigma_at_dmax": float(math.sqrt(v[i]) / m[i])}
+    out["fp32_over_fp64"] = out["fp32"]["histories_per_s"] / out["fp64"]["histories_per_s"]
+    out["accuracy_fp32_vs_fp64"] = equal_accuracy(("mv", res["fp32"][0], res["fp32"][1], 5 * (nbatch - 1)),
+                                                  ("mv", res["fp64"][0], res["fp64"][1], 5 * (nbatch - 1)), rho=rho)
+    from ompmc_amd import problem as P
+    P.free_problem(prob)
     return out
 
 
 # ------------------------------------------------------------ CPU baseline --
+def host_info():
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    libc = " ".join(platform.libc_ver())
+    return {"cpu_model": model, "host_cpus": os.cpu_count(), "affinity_cpus": len(os.sched_getaffinity(0)),
+            "glibc": libc}
+
+
 def cpu_baseline(stem: Path, workdir: Path, ncase: int):
     """The reference's own OpenMP build (oracle/_ref) on a bounded sample of
-    the same workload; the oracle's CPU restatement when that is absent."""
-    cores = min(len(os.sched_getaffinity(0)), 16)
+    the same workload, on the CPU share this process has: the cores in its
+    affinity mask, limited by OMP_NUM_THREADS where the host sets it (a GPU
+    box gives each GPU a share of its cores).  The oracle's CPU restatement
+    when the reference build is absent."""
+    info = host_info()
+    cores = info["affinity_cpus"]
+    if os.environ.get("OMP_NUM_THREADS", "").isdigit():
+        cores = min(cores, int(os.environ["OMP_NUM_THREADS"]))
     exe = ROOT / "oracle" / "_ref" / "omc_dosxyz_ref_omp"
     text = Path(str(stem) + ".inp").read_text()
     text = re.sub(r"ncase = \d+", f"ncase = {ncase}", text)
     sample = workdir / "cpu_sample"
     Path(str(sample) + ".inp").write_text(text)
     if exe.exists():
-        cmd = ["taskset", "-c", f"0-{cores - 1}", str(exe), "-i", str(sample), "-o", "cpu_sample"]
+        cmd = ["taskset", "-c", ",".join(str(c) for c in sorted(os.sched_getaffinity(0))[:cores]), str(exe),
+               "-i", str(sample), "-o", "cpu_sample"]
         t0 = time.perf_counter()
         r = subprocess.run(cmd, capture_output=True, text=True, timeout=600)
         wall = time.perf_counter() - t0
@@ -396,8 +693,8 @@ def cpu_baseline(stem: Path, workdir: Path, ncase: int):
         transport = ts[1] - ts[0] if len(ts) >= 2 else wall
         return {"value": ncase / transport, "unit": "histories/s", "cores": cores, "kind": "reference",
                 "sample": f"{ncase} histories of the same slab/6 MV workload; reference omc_dosxyz "
-                          f"(gcc -O2 -mtune=native -fopenmp -fcommon), {cores} OpenMP threads, transport "
-                          f"{transport:.2f} s of {wall:.2f} s total"}
+                          f"(gcc -O2 -mtune=native -fopenmp -fcommon), {cores} OpenMP threads pinned with taskset, "
+                          f"transport {transport:.2f} s of {wall:.2f} s total", **info}
     sys.path.insert(0, str(ROOT / "oracle"))
     import pyoracle as O
     prob = O.load_problem(sample)
@@ -406,7 +703,7 @@ def cpu_baseline(stem: Path, workdir: Path, ncase: int):
     el = time.perf_counter() - t0
     O.free_problem(prob)
     return {"value": ncase / el, "unit": "histories/s", "cores": cores, "kind": "port",
-            "sample": f"{ncase} histories, CPU restatement with {cores} OpenMP threads"}
+            "sample": f"{ncase} histories, CPU restatement with {cores} OpenMP threads", **info}
 
 
 def main():
@@ -415,14 +712,10 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--hist", type=int, default=100_000_000)
-    ap.add_argument("--pipelines", type=int, default=0)
-    ap.add_argument("--chunk", type=int, default=0)
     ap.add_argument("--cpu-sample", type=int, default=10_000_000)
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-fp64", action="store_true")
-    ap.add_argument("--no-beamlets", action="store_true")
-    ap.add_argument("--no-latency", action="store_true")
-    ap.add_argument("--no-tune", action="store_true")
+    ap.add_argument("--c5-hist", type=int, default=1_000_000_000)
+    for sec in ("cpu", "fp64", "beamlets", "latency", "tune", "c4", "c5"):
+        ap.add_argument(f"--no-{sec}", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this engine runs on one GPU per process; --gpus must be 1 "
@@ -432,10 +725,11 @@ def main():
     from ompmc_amd import device as D
     work = Path(tempfile.mkdtemp(prefix="omc_bench_"))
     prob, stem = slab_problem(work, args.hist)
-    log(f"slab FP32: {args.steps} steps of {args.hist} histories")
-    el32, st32, e32, nh, b32 = time_slab(prob, D.FP32, args.hist, args.steps, args.warmup, args.pipelines,
-                                         args.chunk)
+    log(f"slab FP32: tune, then {args.steps} steps of {args.hist} histories")
+    el32, st32, e32, nh, b32, tune32 = time_slab(prob, D.FP32, args.hist, args.steps, args.warmup,
+                                                 tune=not args.no_tune)
     hps32 = args.steps * nh / el32
+    ms32 = 1000.0 * el32 / args.steps
     out = {
         "metric": METRIC,
         "value": hps32,
@@ -443,7 +737,7 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": 1000.0 * el32 / args.steps,
+        "ms_per_step": ms32,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -451,39 +745,58 @@ def main():
         "data": "synthetic (phantoms, spectrum and beamlets generated in-process)",
         "config": {"workload": "slab phantom 64^3 @ 0.25 cm (water/lung 0.26/bone 1.85/water), synthetic 6 MV, "
                                "10x10 cm2 at SSD 100, ECUT 0.7 MeV, PCUT 0.01 MeV; FP32 transport, FP64 tallies",
-                   "histories_per_step": nh, "nbatch": 10,
+                   "histories_per_step": nh, "nbatch": 50,
+                   "engine_settings": "autotuned on the same engine before the timed steps" if tune32 else
+                                      "engine defaults",
                    "photon_flights": {"fp32": "delta tracking (engine default)",
                                       "fp64": "voxel stepping, as the reference (engine default)"}},
-        "roofline": roofline(st32, 1),
     }
+    oracle_units = None
     if not args.no_fp64:
-        log("slab FP64")
-        el64, st64, e64, _, b64 = time_slab(prob, D.FP64, args.hist, args.steps, args.warmup, args.pipelines,
-                                            args.chunk, seed0=3000)
+        log("slab FP64: tune, then the same steps")
+        el64, st64, e64, _, b64, tune64 = time_slab(prob, D.FP64, args.hist, args.steps, args.warmup,
+                                                    tune=not args.no_tune, seed0=3000)
         hps64 = args.steps * nh / el64
-        out["fp64"] = {"value": hps64, "ms_per_step": 1000.0 * el64 / args.steps, "roofline": roofline(st64, 0)}
+        ms64 = 1000.0 * el64 / args.steps
+        t64 = {k: sum(s[k] for s in st64) for k in st64[0]}
+        # the FP64 engine steps photons voxel by voxel and replays the
+        # oracle's histories exactly: its counts are the reference's units
+        oracle_units = {"deposits": t64["deposits"] / t64["histories"],
+                        "crossings": (t64["photon_crossings"] + t64["electron_crossings"]) / t64["histories"]}
+        out["fp64"] = {"value": hps64, "ms_per_step": ms64, "roofline": roofline(st64, 0, ms64, args.steps),
+                       "autotune": tune64}
         out["fp32_over_fp64"] = hps32 / hps64
         out["energy_per_history_MeV"] = {"fp32": float(np.mean(e32)) / nh, "fp64": float(np.mean(e64)) / nh}
-        out["accuracy_fp32_vs_fp64"] = accuracy_gates(prob, b32, b64)
+        out["accuracy_fp32_vs_fp64"] = equal_accuracy(b32, b64, rho=density(prob))
+    out["roofline"] = roofline(st32, 1, ms32, args.steps, oracle_units)
+    out["autotune"] = tune32
     out["device_stats_fp32_last_step"] = st32[-1]
-    if not args.no_tune:
-        log("autotune FP32 on the slab workload")
-        out["autotune"] = autotune_section(prob, D.FP32, probe=min(args.hist, 20_000_000))
-    if not args.no_beamlets or not args.no_latency:
-        pprob, src, bl_nh = patient_problem(work)
+    if not (args.no_beamlets and args.no_latency):
+        pprob, src = patient_problem(work)
+        from ompmc_amd import configs as cfg
+        bl_nh = cfg.loguniform_histories(src.nbeamlets, 1e4, 1e6, seed=2024)
         if not args.no_beamlets:
             res = {}
             for name, prec in (("fp32", D.FP32), ("fp64", D.FP64)):
-                log(f"beamlets {name}: {src.nbeamlets} beamlets, {int(bl_nh.sum())} histories")
-                res[name], _ = beamlet_batch(pprob, src, bl_nh, prec)
+                log(f"beamlets {name}: {src.nbeamlets} beamlets, {int(bl_nh.sum())} histories (tuned)")
+                res[name] = beamlet_batch(pprob, src, bl_nh, prec, tune=not args.no_tune)
             log("beamlets fp32, photon flights stepped voxel by voxel as in the reference")
-            res["fp32_voxel_stepping"], _ = beamlet_batch(pprob, src, bl_nh, D.FP32, engine={"woodcock": -1})
+            res["fp32_voxel_stepping"] = beamlet_batch(pprob, src, bl_nh, D.FP32, engine={"woodcock": -1})
             log(f"beamlets plain fp64: {src.nbeamlets} jobs in turn")
             res["plain_fp64"] = beamlet_plain(pprob, src, bl_nh, D.FP64)
-            res["fp32_over_fp64"] = res["fp32"]["histories_per_s"] / res["fp64"]["histories_per_s"]
-            res["fp32_batched_over_plain_fp64"] = res["fp32"]["histories_per_s"] / res["plain_fp64"]["histories_per_s"]
-            res["fp32_voxel_stepping_batched_over_plain_fp64"] = (res["fp32_voxel_stepping"]["histories_per_s"] /
-                                                                  res["plain_fp64"]["histories_per_s"])
+            log("beamlets: FP32 vs FP64 equal-accuracy gate")
+            res["accuracy_fp32_vs_fp64"] = beamlet_gate(pprob, src, bl_nh)
+            r32, r64, rp = (res[k]["histories_per_s"] for k in ("fp32", "fp64", "plain_fp64"))
+            res["fp32_batched_over_fp64_batched"] = r32 / r64
+            res["fp32_batched_over_plain_fp64"] = r32 / rp
+            res["fp32_voxel_stepping_batched_over_plain_fp64"] = res["fp32_voxel_stepping"]["histories_per_s"] / rp
+            res["target_3x"] = {
+                "against_plain_fp64_one_beamlet_per_job": bool(r32 / rp >= 3.0),
+                "against_fp64_batched_in_one_pass": bool(r32 / r64 >= 3.0),
+                "equal_accuracy_gate": res["accuracy_fp32_vs_fp64"]["pass"],
+                "note": "the target is met against the plain FP64 method (each beamlet its own job, in turn); "
+                        "against the FP64 engine batching all beamlets in one pass the ratio is the "
+                        "precision gain alone"}
             res["workload"] = ("patient 128x128x96 @ 0.25 cm, 400 beamlets 0.5x0.5 cm2 at gantry 0, histories "
                                "log-uniform [1e4, 1e6] per beamlet, nbatch 10, sparse Dij at 1% of beamlet max; "
                                "fp32 = delta-tracked photon flights (engine default), fp32_voxel_stepping and the "
@@ -491,9 +804,14 @@ def main():
             out["beamlets"] = res
         if not args.no_latency:
             out["latency"] = latency_section(pprob, src)
+    if not args.no_c4:
+        out["c4"] = c4_section(work)
+    if not args.no_c5:
+        out["c5"] = c5_section(work, hist=args.c5_hist)
     if not args.no_cpu:
         log(f"CPU baseline: {args.cpu_sample} histories")
         out["cpu_baseline"] = cpu_baseline(stem, work, args.cpu_sample)
+        out["gpu_over_cpu_baseline"] = hps32 / out["cpu_baseline"]["value"]
     print(json.dumps(out))
 
 

@@ -105,6 +105,18 @@ typedef struct {
     int64_t woodcock_over;      /* delta-tracking sites whose attenuation
                                    exceeded the majorant (0 unless the
                                    majorant is wrong) */
+    /* Physics events of the run, counted where the reference calls the
+     * routine named: with the same histories (FP64 engine, voxel-stepped
+     * photons) each equals the count of the CPU restatement exactly. */
+    int64_t compton, photo, pair, rayleigh;          /* ompmc.c:2046-2065 */
+    int64_t brems, moller, bhabha;                   /* ompmc.c:5375-5429 */
+    int64_t annih_flight;       /* annih(), positron in flight */
+    int64_t annih_rest;         /* rannih(): positron cut off or leaving */
+    int64_t fict_rejections;    /* discrete events rejected by sigf/sig0
+                                   (ompmc.c:5354-5372) */
+    int64_t msdist;             /* condensed-history steps (msdist calls) */
+    int64_t sscat;              /* single elastic scatterings (sscat calls) */
+    int64_t howfar_charged;     /* howfar calls for charged particles */
 } omc_dev_stats;
 
 /* Context: device, pipelines (each a stream with its own particle queues),
@@ -155,9 +167,10 @@ int  omc_dev_run_dosxyz(omc_dev_ctx *ctx, const omc_source_dosxyz *src,
  * nperbatch_b * rp->nbatch histories with nperbatch_b = nhist[b] / nbatch
  * (1 if that is 0, as omc_matrad.c:1332-1337 does for its single count);
  * its histories take stream indices after those of beamlets 0..b-1,
- * starting at rp->hist0.  rp->nperbatch and rp->ledger are not used.
+ * starting at rp->hist0.  rp->nperbatch is not used.
  * sum [b*nreg + i] and sum2[b*nreg + i] are increased by the batch sums,
- * as omc_dev_run_dosxyz does for one field. */
+ * as omc_dev_run_dosxyz does for one field.  rp->ledger, if set, receives
+ * 3 entries per history of the run, beamlet after beamlet. */
 int  omc_dev_run_beamlets(omc_dev_ctx *ctx, const omc_source_beamlets *src,
                           const int64_t *nhist, const omc_dev_run_params *rp,
                           double *sum, double *sum2);
@@ -191,6 +204,12 @@ int  omc_dev_beamlets_sparse(omc_dev_ctx *ctx, const omc_source_beamlets *src,
  * and *hps the histories/s measured for them.  No work may be queued. */
 int  omc_dev_tune(omc_dev_ctx *ctx, const omc_source_dosxyz *src, const omc_dev_run_params *rp,
                   int64_t probe, omc_dev_opts *best, double *hps);
+
+/* The same autotuner on a matRad beamlet batch: times omc_dev_run_beamlets of
+ * `src` with nhist[b] histories per beamlet (the caller's probe, e.g. the
+ * batch itself or a subset) under each setting; no dose is copied back. */
+int  omc_dev_tune_beamlets(omc_dev_ctx *ctx, const omc_source_beamlets *src, const int64_t *nhist,
+                           const omc_dev_run_params *rp, omc_dev_opts *best, double *hps);
 
 /* ---- serving: continuous batching of dose requests ----------------------- */
 typedef struct {

@@ -249,3 +249,50 @@ def beamlets(gantry_deg=(0.0,), field=10.0, bixel=0.5, sad=100.0, iso=(0.0, 0.0,
 def loguniform_histories(n, lo, hi, seed=12345):
     rng = np.random.default_rng(seed)
     return np.exp(rng.uniform(math.log(lo), math.log(hi), size=n)).astype(np.int64)
+
+
+def c5_beams(sad=100.0, field=8.0, gantry=(0.0, 72.0, 144.0, 216.0, 288.0)) -> BeamletSet:
+    """The large multi-beam plan: five coplanar beams, each one field x field
+    square at the isocentre (one beamlet per beam, matRad beam geometry)."""
+    return beamlets(gantry_deg=gantry, field=field, bixel=field, sad=sad)
+
+
+def c4_phantoms():
+    """The three phantoms the serving stream keeps resident -- water 40^3 at
+    0.5 cm, the 64^3 slab, the 128x128x96 patient -- on one media list
+    (AIR, LUNG, WATER, BONE), so one table upload serves all three."""
+    names = [AIR, LUNG, WATER, BONE]
+    out = []
+    for ph in (water_cube(40, 20.0), slab_phantom(64, 16.0), patient_phantom()):
+        lut = np.array([0] + [names.index(n) + 1 for n in ph.names], dtype=np.int32)
+        ph.med = lut[ph.med]
+        ph.names = list(names)
+        out.append(ph)
+    return out
+
+
+@dataclass
+class ServingRequest:
+    arrival_s: float         # time after the stream starts
+    phantom: int             # 0 water 40^3, 1 slab 64^3, 2 patient 128x128x96
+    side_cm: float           # square field at the phantom surface (SSD 100)
+    energy: float            # mono energy in MeV, or 0 for the 6 MV spectrum
+    nhist: int
+
+
+def serving_stream(n, rate, seed=4242, hist_lo=1e5, hist_hi=1e8):
+    """The serving stream: Poisson arrivals at `rate` per second, phantom
+    uniform over the three resident ones, square field side U(1, 20) cm,
+    energy mono U(1, 6) MeV or the 6 MV spectrum with probability 1/2 each,
+    histories log-uniform in [hist_lo, hist_hi]."""
+    rng = np.random.default_rng(seed)
+    t = np.cumsum(rng.exponential(1.0 / rate, size=n))
+    out = []
+    for i in range(n):
+        ph = int(rng.integers(0, 3))
+        side = float(rng.uniform(1.0, 20.0))
+        energy = float(rng.uniform(1.0, 6.0)) if rng.random() < 0.5 else 0.0
+        nh = int(math.exp(rng.uniform(math.log(hist_lo), math.log(hist_hi))))
+        out.append(ServingRequest(float(t[i]), ph, side, energy, nh))
+    return out
+

@@ -318,6 +318,17 @@ __device__ __forceinline__ void q_store(const Queue<R> &q, uint32_t i, const Par
     q.k0[i] = p.rng.k0; q.k1[i] = p.rng.k1; q.ctr[i] = p.rng.ctr;
 }
 
+// Physics events counted by the transport kernels, in the order of the
+// omc_dev_stats fields.  The rare ones (interactions) are counted per block
+// with LDS atomics; the frequent ones (every condensed history step) in a
+// per-thread LDS slot, so neither holds a register.
+enum EvKind {
+    EV_COMPTON, EV_PHOTO, EV_PAIR, EV_RAYLEIGH, EV_BREMS, EV_MOLLER, EV_BHABHA, EV_ANNIH, EV_RANNIH, EV_FICT,
+    EV_NRARE,
+    EV_MSDIST = EV_NRARE, EV_SSCAT, EV_HOWFAR_E,
+    EV_N
+};
+
 // Counters and error flags shared by one launch.
 struct DevStatus {
     uint32_t overflow;       // a push found its queue full
@@ -327,6 +338,51 @@ struct DevStatus {
     unsigned long long photon_cross, electron_cross;   // voxel boundaries crossed
     unsigned long long photon_dep, electron_dep;       // ausgab calls
     unsigned long long wc_over;                        // delta-tracking sites above the majorant
+    unsigned long long ev[EV_N];                       // physics events (EvKind)
 };
+
+static __shared__ uint32_t ev_block[EV_NRARE];
+static __shared__ uint32_t ev_lane[EV_N - EV_NRARE][RNG_LANES];
+
+// Zeroes the block's event counters (the per-thread slots only where the
+// kernel counts frequent events); the caller synchronises the block before
+// the first count.
+template <bool LANES> __device__ __forceinline__ void events_begin()
+{
+    if (threadIdx.x < EV_NRARE) ev_block[threadIdx.x] = 0u;
+    if (LANES)
+        for (int k = 0; k < EV_N - EV_NRARE; ++k) ev_lane[k][threadIdx.x] = 0u;
+}
+
+// A rare event in this lane: an LDS atomic (same-address lanes of a
+// wavefront are serialised by the LDS, which costs little for events that
+// come a few times per history).
+__device__ __forceinline__ void count_event(int k) { atomicAdd(&ev_block[k], 1u); }
+
+// A rare event in every active lane, straight into the launch status: one
+// global atomic per wavefront (kernels that keep no block counters).
+__device__ __forceinline__ void count_event_global(DevStatus *st, int k)
+{
+    const uint64_t m = __ballot(1);
+    if (lane_id() == __ffsll((unsigned long long)m) - 1) atomicAdd(&st->ev[k], (unsigned long long)__popcll(m));
+}
+
+// A frequent event of this thread's lane.
+__device__ __forceinline__ void count_lane_event(int k) { ev_lane[k - EV_NRARE][threadIdx.x] += 1u; }
+
+// Adds the block's counts to the launch status (every thread of the block
+// calls it once, after its transport loop).
+template <bool LANES> __device__ __forceinline__ void events_end(DevStatus *st)
+{
+    __syncthreads();
+    const int k = threadIdx.x;
+    if (k < EV_NRARE) {
+        if (ev_block[k]) atomicAdd(&st->ev[k], (unsigned long long)ev_block[k]);
+    } else if (LANES && k < EV_N) {
+        unsigned long long n = 0;
+        for (int t = 0; t < RNG_LANES; ++t) n += ev_lane[k - EV_NRARE][t];
+        if (n) atomicAdd(&st->ev[k], n);
+    }
+}
 
 }  // namespace omc

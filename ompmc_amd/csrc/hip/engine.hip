@@ -24,6 +24,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <deque>
+#include <functional>
 #include <map>
 #include <thread>
 #include <vector>
@@ -160,6 +161,7 @@ struct Work {
     int nbatch = 1;
     int64_t nreg = 0;
     bool ledger = false;
+    int64_t ledger_off = 0;       // first history of this work in the run's ledger
     int state = W_WAITING;
     int status = OMC_OK;
     int job = -1;
@@ -171,7 +173,7 @@ struct Work {
 struct RunSink {
     bool beamlets = false;
     double *sum = nullptr, *sum2 = nullptr;   // [nreg] or [nb*nreg]
-    double *ledger = nullptr;                 // dosxyz only
+    double *ledger = nullptr;                 // [3 * histories of the run]
     bool sparse = false;
     double rel_thr = 0.0;
     std::vector<std::vector<int64_t>> rows;
@@ -225,6 +227,7 @@ struct omc_dev_ctx {
     double h_ge0[MAXMED] = {}, h_ge1[MAXMED] = {};
     void *small = nullptr, *ph_mfp = nullptr, *ph_br = nullptr, *ph_pmax = nullptr;
     void *el = nullptr, *csda = nullptr, *spin = nullptr, *ms = nullptr;
+    size_t tab_bytes[8] = {};         // sizes of the eight table buffers above, in that order
     // phantoms
     PhantomSlot ph[MAX_PHANTOMS];
     void *d_phantoms = nullptr;       // DevPhantom<R>[MAX_PHANTOMS]
@@ -499,9 +502,11 @@ template <typename R> static int upload_tables(omc_dev_ctx *c, const omc_tables 
         { &c->spin, spin.data(), spin.size() * sizeof(spin[0]) },
         { &c->ms, ms.data(), ms.size() * sizeof(ms[0]) },
     };
-    for (auto &u : up) {
-        if (*u.dst) { (void)hipFree(*u.dst); *u.dst = nullptr; }
+    for (int b = 0; b < 8; ++b) {
+        auto &u = up[b];
+        if (*u.dst) { dfree(c, *u.dst, c->tab_bytes[b]); *u.dst = nullptr; c->tab_bytes[b] = 0; }
         OMC_TRY_DEV(dalloc(c, u.dst, u.bytes));
+        c->tab_bytes[b] = u.bytes;
         HIPCHK(hipMemcpy(*u.dst, u.src, u.bytes, hipMemcpyHostToDevice));
     }
     c->media = true;
@@ -822,7 +827,7 @@ static int finish_deliver(omc_dev_ctx *c, int j)
         }
     }
     if (k && k->ledger && J.ledger &&
-        hipMemcpy(k->ledger, J.ledger, J.ledger_bytes, hipMemcpyDeviceToHost) != hipSuccess)
+        hipMemcpy(k->ledger + 3 * w->ledger_off, J.ledger, J.ledger_bytes, hipMemcpyDeviceToHost) != hipSuccess)
         return fail(OMC_EDEVICE, "copy ledger");
     if (k && k->sparse) {
         k->rows[w->index].assign(c->h_rows, c->h_rows + J.nnz);
@@ -899,7 +904,14 @@ static int pull_segments(omc_dev_ctx *c, int pi, int64_t room, uint32_t &injP, u
         c->runnable.pop_front();
         JobRec &J = c->jobs[j];
         int64_t left = J.w->nhist - J.injected;
-        int64_t take = std::min<int64_t>(std::min<int64_t>(left, room), J.quantum);
+        // A primary photon of a splitting job is followed as nsplit
+        // sub-photons in one lane, each setting up to two charged particles
+        // in motion, all pushed before the lane lets go of it: it counts as
+        // nsplit particles against the room, so one generation's secondaries
+        // stay within the queues (four times the chunk).  One primary at a
+        // time still fits the smallest queue (4096 >= 2 * OMC_MAX_NSPLIT).
+        const int64_t weight = J.w->tmpl.nsplit > 1 ? J.w->tmpl.nsplit : 1;
+        int64_t take = std::min<int64_t>(std::min<int64_t>(left, std::max<int64_t>(room / weight, 1)), J.quantum);
         SourceSeg &sg = P.h->segs[nseg++];
         sg.first = (uint64_t)J.injected;
         sg.job = (uint32_t)j;
@@ -907,7 +919,7 @@ static int pull_segments(omc_dev_ctx *c, int pi, int64_t room, uint32_t &injP, u
         sg.off = off;
         sg.pad = 0;
         off += (uint32_t)take;
-        room -= take;
+        room -= take * weight;
         J.injected += take;
         if (J.w->tmpl.charge == 0) injP += (uint32_t)take; else injE += (uint32_t)take;
         if (!(J.pipes & (1u << pi))) {
@@ -1173,6 +1185,10 @@ static void stats_end(omc_dev_ctx *c, omc_dev_stats *out, int64_t histories, dou
         S.killed += (int64_t)(P.h->status.killed - P.base.killed);
         S.woodcock_over += (int64_t)(P.h->status.wc_over - P.base.wc_over);
         S.yields += (int64_t)(P.h->status.yields - P.base.yields);
+        int64_t *ev[EV_N] = { &S.compton, &S.photo, &S.pair, &S.rayleigh, &S.brems, &S.moller, &S.bhabha,
+                              &S.annih_flight, &S.annih_rest, &S.fict_rejections, &S.msdist, &S.sscat,
+                              &S.howfar_charged };
+        for (int k = 0; k < EV_N; ++k) *ev[k] += (int64_t)(P.h->status.ev[k] - P.base.ev[k]);
     }
     S.deposits = S.photon_deposits + S.electron_deposits;
     S.histories = histories;
@@ -1480,37 +1496,29 @@ int apply_setting(omc_dev_ctx *c, const TuneSetting &t)
 
 }  // namespace
 
-int omc_dev_tune(omc_dev_ctx *c, const omc_source_dosxyz *src, const omc_dev_run_params *rp, int64_t probe,
-                 omc_dev_opts *best, double *hps)
+// Coordinate descent over the engine settings (one option at a time, from
+// the current settings, one pass); each setting is timed as the better of
+// two calls of `run` (which runs `hist` histories).  Leaves the context with
+// the fastest settings and reports them.
+static int tune_impl(omc_dev_ctx *c, const std::function<int()> &run, int64_t hist, omc_dev_opts *best,
+                     double *hps)
 {
-    OMC_TRY_DEV(check_run(c, rp));
-    if (!src || !best || !hps) return fail(OMC_EARG, "null argument");
-    if (probe < rp->nbatch) return fail(OMC_EARG, "probe of %lld histories for %d batches", (long long)probe, rp->nbatch);
-    if (any_running(c) || !c->waiting.empty()) return fail(OMC_EARG, "the autotuner needs a context with no queued work");
-    HIPCHK(hipSetDevice(c->device));
     int max_pipes = MAX_PIPES;
     if (const char *q = getenv("GPU_MAX_HW_QUEUES")) {
         const int n = atoi(q);
         if (n > 1) max_pipes = std::min(max_pipes, n - 1);
     }
-    omc_dev_run_params r = *rp;
-    r.nperbatch = probe / rp->nbatch;
-    r.ledger = nullptr;
-    r.stats = nullptr;
-    const int nreg = c->ph[rp->phantom_id].nreg;
-    std::vector<double> s1(nreg), s2(nreg);
     auto measure = [&](const TuneSetting &t, double &v) -> int {
         OMC_TRY_DEV(apply_setting(c, t));
         v = 0;
         for (int k = 0; k < 2; ++k) {
             const auto t0 = std::chrono::steady_clock::now();
-            OMC_TRY_DEV(omc_dev_run_dosxyz(c, src, &r, s1.data(), s2.data()));
+            OMC_TRY_DEV(run());
             const double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-            v = std::max(v, (double)(r.nperbatch * r.nbatch) / std::max(sec, 1e-9));
+            v = std::max(v, (double)hist / std::max(sec, 1e-9));
         }
         return OMC_OK;
     };
-    // one option at a time, from the current settings (coordinate descent)
     static const std::vector<int64_t> axes[7] = {
         { 2, 4, 6, 8 },                              // pipelines
         { 1 << 20, 1 << 21, 1 << 22, 1 << 23 },      // live particles per pipeline
@@ -1565,6 +1573,24 @@ int omc_dev_tune(omc_dev_ctx *c, const omc_source_dosxyz *src, const omc_dev_run
     return OMC_OK;
 }
 
+int omc_dev_tune(omc_dev_ctx *c, const omc_source_dosxyz *src, const omc_dev_run_params *rp, int64_t probe,
+                 omc_dev_opts *best, double *hps)
+{
+    OMC_TRY_DEV(check_run(c, rp));
+    if (!src || !best || !hps) return fail(OMC_EARG, "null argument");
+    if (probe < rp->nbatch) return fail(OMC_EARG, "probe of %lld histories for %d batches", (long long)probe, rp->nbatch);
+    if (any_running(c) || !c->waiting.empty()) return fail(OMC_EARG, "the autotuner needs a context with no queued work");
+    HIPCHK(hipSetDevice(c->device));
+    omc_dev_run_params r = *rp;
+    r.nperbatch = probe / rp->nbatch;
+    r.ledger = nullptr;
+    r.stats = nullptr;
+    const int nreg = c->ph[rp->phantom_id].nreg;
+    std::vector<double> s1(nreg), s2(nreg);
+    return tune_impl(c, [&] { return omc_dev_run_dosxyz(c, src, &r, s1.data(), s2.data()); },
+                     r.nperbatch * r.nbatch, best, hps);
+}
+
 static int beamlet_works(omc_dev_ctx *c, const omc_source_beamlets *src, const int64_t *nhist,
                          const omc_dev_run_params *rp, std::vector<Work> &works)
 {
@@ -1573,7 +1599,6 @@ static int beamlet_works(omc_dev_ctx *c, const omc_source_beamlets *src, const i
     if (src->nbeamlets < 1 || !src->ibeam || !src->src || !src->corner || !src->side1 || !src->side2)
         return fail(OMC_EARG, "empty beamlet source");
     if (src->charge < -1 || src->charge > 1) return fail(OMC_EARG, "charge %d", src->charge);
-    if (rp->ledger) return fail(OMC_EARG, "the per-history ledger is only available for dosxyz runs");
     works.resize(src->nbeamlets);
     int64_t h0 = rp->hist0;
     for (int b = 0; b < src->nbeamlets; ++b) {
@@ -1583,6 +1608,8 @@ static int beamlet_works(omc_dev_ctx *c, const omc_source_beamlets *src, const i
         OMC_TRY_DEV(set_histories(w, nhist[b] / rp->nbatch, rp->nbatch));
         stream_fields(w, rp->seed, h0);
         w.tmpl.nsplit = rp->nsplit;
+        w.ledger = rp->ledger != nullptr;
+        w.ledger_off = h0 - rp->hist0;
         h0 += w.nhist;
         w.index = b;
         w.phantom = rp->phantom_id;
@@ -1602,6 +1629,7 @@ int omc_dev_run_beamlets(omc_dev_ctx *c, const omc_source_beamlets *src, const i
     k.beamlets = true;
     k.sum = sum;
     k.sum2 = sum2;
+    k.ledger = rp->ledger;
     return run_sink(c, k, works, rp->stats);
 }
 
@@ -1613,6 +1641,7 @@ int omc_dev_beamlets_sparse(omc_dev_ctx *c, const omc_source_beamlets *src, cons
     if (src) {
         std::vector<Work> works;
         OMC_TRY_DEV(beamlet_works(c, src, nhist, rp, works));
+        for (auto &w : works) w.ledger = false;       // the Dij call returns no ledger
         HIPCHK(hipSetDevice(c->device));
         RunSink k;
         k.beamlets = true;
@@ -1643,6 +1672,28 @@ int omc_dev_beamlets_sparse(omc_dev_ctx *c, const omc_source_beamlets *src, cons
     c->kept_rows.clear();
     c->kept_vals.clear();
     return OMC_OK;
+}
+
+int omc_dev_tune_beamlets(omc_dev_ctx *c, const omc_source_beamlets *src, const int64_t *nhist,
+                          const omc_dev_run_params *rp, omc_dev_opts *best, double *hps)
+{
+    std::vector<Work> probe;
+    OMC_TRY_DEV(beamlet_works(c, src, nhist, rp, probe));      // validates the arguments
+    if (!best || !hps) return fail(OMC_EARG, "null argument");
+    if (any_running(c) || !c->waiting.empty()) return fail(OMC_EARG, "the autotuner needs a context with no queued work");
+    HIPCHK(hipSetDevice(c->device));
+    int64_t hist = 0;
+    for (auto &w : probe) hist += w.nhist;
+    omc_dev_run_params r = *rp;
+    r.stats = nullptr;
+    return tune_impl(c, [&] {
+        std::vector<Work> works;
+        OMC_TRY_DEV(beamlet_works(c, src, nhist, &r, works));
+        for (auto &w : works) w.ledger = false;
+        RunSink k;
+        k.beamlets = true;     // no sums wanted: only the transport is timed
+        return run_sink(c, k, works, nullptr);
+    }, hist, best, hps);
 }
 
 int omc_dev_submit(omc_dev_ctx *c, const omc_dev_request *req)

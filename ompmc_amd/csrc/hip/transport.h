@@ -228,6 +228,7 @@ __device__ __forceinline__ bool split_interaction(const KArgs<R> &A, Particle<R>
     R rnno = urand<R>(q.rng);
     if (rnno <= (R)1.0 - s.cohfac) {
         if (survivor) {
+            count_event_global(A.status, EV_RAYLEIGH);
             q.wt *= (R)nsplit;
             rayleigh<R>(T, q, imed, s.eig, s.gle, s.lgle);
             p.rng = q.rng;
@@ -243,12 +244,14 @@ __device__ __forceinline__ bool split_interaction(const KArgs<R> &A, Particle<R>
     R gbr1 = s.gle * br.a1 + br.a0;
     Particle<R> b;
     if (rnno <= gbr1 && s.eig > (R)2.0 * (R)RM) {
+        count_event_global(A.status, EV_PAIR);
         pair<R>(T, q, b, imed);
         p.rng = q.rng;
         detach_stream<R>(p, q);
         push<R>(A.qe, q, A.status);
         push<R>(A.qe, b, A.status);
     } else if (rnno < s.gle * br.b1 + br.b0) {
+        count_event_global(A.status, EV_COMPTON);
         compton<R>(q, b);
         p.rng = q.rng;
         push<R>(A.qe, b, A.status);
@@ -258,6 +261,7 @@ __device__ __forceinline__ bool split_interaction(const KArgs<R> &A, Particle<R>
             push<R>(A.qp, q, A.status);
         }
     } else {
+        count_event_global(A.status, EV_PHOTO);
         photo<R>(q, (R)G.g->ecut[imed + 1]);
         p.rng = q.rng;
         detach_stream<R>(p, q);
@@ -287,6 +291,7 @@ __device__ __forceinline__ bool photon_event(const KArgs<R> &A, Particle<R> &p, 
         const int imed = s.imed;
         R rnno = urand<R>(p.rng);
         if (rnno <= (R)1.0 - s.cohfac) {
+            count_event_global(A.status, EV_RAYLEIGH);
             rayleigh<R>(T, p, imed, s.eig, s.gle, s.lgle);
         } else {
             rnno = urand<R>(p.rng);
@@ -294,6 +299,7 @@ __device__ __forceinline__ bool photon_event(const KArgs<R> &A, Particle<R> &p, 
             R gbr1 = s.gle * br.a1 + br.a0;
             Particle<R> b;
             if (rnno <= gbr1 && s.eig > (R)2.0 * (R)RM) {
+                count_event_global(A.status, EV_PAIR);
                 pair<R>(T, p, b, imed);
                 push<R>(A.qe, p, A.status);
                 push<R>(A.qe, b, A.status);
@@ -301,9 +307,11 @@ __device__ __forceinline__ bool photon_event(const KArgs<R> &A, Particle<R> &p, 
             }
             R gbr2 = s.gle * br.b1 + br.b0;
             if (rnno < gbr2) {
+                count_event_global(A.status, EV_COMPTON);
                 compton<R>(p, b);
                 push<R>(A.qe, b, A.status);
             } else {
+                count_event_global(A.status, EV_PHOTO);
                 photo<R>(p, (R)G.g->ecut[imed + 1]);
                 push<R>(A.qe, p, A.status);
                 return false;
@@ -585,6 +593,7 @@ __device__ __forceinline__ void electron_cutoff(const KArgs<R> &A, Particle<R> &
     R edep = p.e - (R)RM;
     sc.add(G, p.ir, (double)p.wt * (double)edep);
     if (p.iq > 0 && edep < eie) {
+        count_event(EV_RANNIH);
         Particle<R> b;
         rannih<R>(p, b);
         push_photon_pair<R>(A, p, b);
@@ -611,7 +620,11 @@ __device__ __forceinline__ int electron_interaction(const KArgs<R> &A, Particle<
         sigf /= dedx0;
         R sigratio = sigf / s.sig0;
         R rfict = urand<R>(p.rng);
-        if (rfict >= sigratio) { s.phase = 1; return E_ALIVE; }
+        if (rfict >= sigratio) {
+            count_event(EV_FICT);
+            s.phase = 1;
+            return E_ALIVE;
+        }
     }
 
     // ---- discrete interaction ----
@@ -623,13 +636,16 @@ __device__ __forceinline__ int electron_interaction(const KArgs<R> &A, Particle<
         R ebr1 = s.elke * eb.br11 + eb.br10;
         R rnno = urand<R>(p.rng);
         if (rnno <= ebr1) {
+            count_event(EV_BREMS);
             brems<R>(T, p, b, m);
             push_made_photon<R>(A, p, b);
         } else if (p.e <= (R)T.small->med[m].thmoll) {
             if (ebr1 <= (R)0) return E_ALIVE;   // neither process possible: re-enter
+            count_event(EV_BREMS);
             brems<R>(T, p, b, m);
             push_made_photon<R>(A, p, b);
         } else {
+            count_event(EV_MOLLER);
             if (moller<R>(T, p, b, m)) push<R>(A.qe, b, A.status);
         }
         return E_ALIVE;
@@ -637,16 +653,19 @@ __device__ __forceinline__ int electron_interaction(const KArgs<R> &A, Particle<
     R pbr1 = s.elke * eb.br11 + eb.br10;
     R rnno = urand<R>(p.rng);
     if (rnno < pbr1) {
+        count_event(EV_BREMS);
         brems<R>(T, p, b, m);
         push_made_photon<R>(A, p, b);
         return E_ALIVE;
     }
     R pbr2 = s.elke * eb.br21 + eb.br20;
     if (rnno < pbr2) {
+        count_event(EV_BHABHA);
         bhabha<R>(T, p, b, m);
         push<R>(A.qe, b, A.status);
         return E_ALIVE;
     }
+    count_event(EV_ANNIH);
     annih<R>(p, b);
     push_photon_pair<R>(A, p, b);
     return E_DONE;
@@ -707,7 +726,10 @@ __device__ __forceinline__ int zero_ustep(const KArgs<R> &A, Particle<R> &p, Ele
     R ustep = tustep;
     int irnew = irl;
     const int irold = irl;
-    if (!(ustep < tperp)) howfar<R>(*G.g, irl, p.x, p.y, p.z, p.u, p.v, p.w, irnew, ustep);
+    if (!(ustep < tperp)) {
+        count_lane_event(EV_HOWFAR_E);
+        howfar<R>(*G.g, irl, p.x, p.y, p.z, p.u, p.v, p.w, irnew, ustep);
+    }
     if (ustep < (R)0) ustep = (R)0.0;
     s.ustep = ustep;
     if (irnew != irold) {
@@ -849,6 +871,7 @@ __device__ __forceinline__ int electron_iteration(const KArgs<R> &A, Particle<R>
             // msdist moves a copy; the original position is kept until
             // after ausgab, which scores in the pre-step region
             Particle<R> q = p;
+            count_lane_event(EV_MSDIST);
             ustep = msdist<R>(T, q, imed, rhof, de, tustep, s.eke);
             p.rng = q.rng;
             xf = q.x; yf = q.y; zf = q.z; uf = q.u; vf = q.v; wf = q.w;
@@ -875,10 +898,12 @@ __device__ __forceinline__ int electron_iteration(const KArgs<R> &A, Particle<R>
 
     int irold = p.ir, irnew = p.ir;
     if (call_howfar) {
+        count_lane_event(EV_HOWFAR_E);
         if (irl == 0) {                      // howfar discard (outside)
             R edep = (iq > 0) ? p.e + rm : p.e - rm;
             sc.add(G, p.ir, (double)p.wt * (double)edep);
             if (iq > 0 && edep < eie) {
+                count_event(EV_RANNIH);
                 Particle<R> b;
                 rannih<R>(p, b);
                 push_photon_pair<R>(A, p, b);
@@ -941,6 +966,7 @@ __device__ __forceinline__ int electron_iteration(const KArgs<R> &A, Particle<R>
             const ElecBin<R> &ebs = T.el[(imed * 2 + qel) * MXEKE + lelkems];
             R etap = elkems * ebs.eta1 + ebs.eta0;
             chia2 *= etap;
+            count_lane_event(EV_SSCAT);
             sscat<R>(T, p.rng, imed, qel, chia2, elkems, beta2, costhe, sinthe);
         } else {
             sinthe = (R)0.0;
@@ -1100,6 +1126,7 @@ template <typename R>
 __global__ void __launch_bounds__(256, Waves<R>::electron) k_electron(KArgs<R> A)
 {
     __shared__ DevPhantom<R> lds_ph[MAX_PHANTOMS];
+    events_begin<true>();
     stage_phantoms<R>(A.phantoms, lds_ph);
     Particle<R> p;
     ElectronLane<R> s;
@@ -1159,6 +1186,7 @@ __global__ void __launch_bounds__(256, Waves<R>::electron) k_electron(KArgs<R> A
     atomicAdd(&A.status->electron_steps, (unsigned long long)nsteps);
     atomicAdd(&A.status->electron_cross, (unsigned long long)ncross);
     atomicAdd(&A.status->electron_dep, (unsigned long long)sc.ndep);
+    events_end<true>(A.status);
 }
 
 }  // namespace omc

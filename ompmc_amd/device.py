@@ -32,7 +32,10 @@ class DevStats(C.Structure):
                 ("electron_launches", C.c_int64), ("killed", C.c_int64), ("photon_crossings", C.c_int64),
                 ("electron_crossings", C.c_int64), ("photon_deposits", C.c_int64),
                 ("electron_deposits", C.c_int64), ("yields", C.c_int64), ("host_ms", C.c_double),
-                ("pipe_idle_ms", C.c_double), ("woodcock_over", C.c_int64)]
+                ("pipe_idle_ms", C.c_double), ("woodcock_over", C.c_int64)] + [
+                (n, C.c_int64) for n in ("compton", "photo", "pair", "rayleigh", "brems", "moller", "bhabha",
+                                         "annih_flight", "annih_rest", "fict_rejections", "msdist", "sscat",
+                                         "howfar_charged")]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -76,6 +79,8 @@ def lib():
         L.omc_dev_run_dosxyz.argtypes = [C.c_void_p, C.POINTER(N.SourceDosxyz), C.POINTER(RunParams), _dp, _dp]
         L.omc_dev_tune.argtypes = [C.c_void_p, C.POINTER(N.SourceDosxyz), C.POINTER(RunParams), C.c_int64,
                                    C.POINTER(DevOpts), C.POINTER(C.c_double)]
+        L.omc_dev_tune_beamlets.argtypes = [C.c_void_p, C.POINTER(N.SourceBeamlets), C.POINTER(C.c_int64),
+                                            C.POINTER(RunParams), C.POINTER(DevOpts), C.POINTER(C.c_double)]
         _i64p = C.POINTER(C.c_int64)
         L.omc_dev_run_beamlets.argtypes = [C.c_void_p, C.POINTER(N.SourceBeamlets), _i64p, C.POINTER(RunParams),
                                            _dp, _dp]
@@ -154,17 +159,33 @@ class Engine:
         check(lib().omc_dev_tune(self.ctx, C.byref(src), C.byref(rp), int(probe), C.byref(best), C.byref(hps)))
         return best.as_dict(), hps.value
 
-    def run_beamlets(self, pid, src, nhist, nbatch, seed=(97, 33), hist0=0, nsplit=1):
-        """Batch sums of every beamlet: arrays (nbeamlets, nreg)."""
+    def tune_beamlets(self, pid, src, nhist, nbatch=10, seed=(97, 33)):
+        """omc_dev_tune_beamlets: the autotuner on a beamlet batch (nhist[b]
+        histories for beamlet b).  Returns (settings dict, histories/s)."""
+        nh = np.ascontiguousarray(nhist, dtype=np.int64)
+        rp = RunParams(pid, 1, (C.c_uint32 * 2)(*seed), 0, 0, nbatch, None, None)
+        best = DevOpts()
+        hps = C.c_double(0)
+        check(lib().omc_dev_tune_beamlets(self.ctx, C.byref(src), nh.ctypes.data_as(C.POINTER(C.c_int64)),
+                                          C.byref(rp), C.byref(best), C.byref(hps)))
+        return best.as_dict(), hps.value
+
+    def run_beamlets(self, pid, src, nhist, nbatch, seed=(97, 33), hist0=0, nsplit=1, ledger=False):
+        """Batch sums of every beamlet: arrays (nbeamlets, nreg); with
+        ledger=True also the per-history ledger (3 values per history,
+        beamlet after beamlet) as a fourth value."""
         nreg = self.nreg[pid]
         nh = np.ascontiguousarray(nhist, dtype=np.int64)
         stats = DevStats()
-        rp = RunParams(pid, nsplit, (C.c_uint32 * 2)(*seed), hist0, 0, nbatch, None, C.pointer(stats))
+        nrun = int(sum(max(1, int(n) // nbatch) * nbatch for n in nh))
+        led = np.zeros(3 * nrun) if ledger else None
+        rp = RunParams(pid, nsplit, (C.c_uint32 * 2)(*seed), hist0, 0, nbatch,
+                       led.ctypes.data_as(_dp) if ledger else None, C.pointer(stats))
         s = np.zeros((src.nbeamlets, nreg))
         s2 = np.zeros((src.nbeamlets, nreg))
         check(lib().omc_dev_run_beamlets(self.ctx, C.byref(src), nh.ctypes.data_as(C.POINTER(C.c_int64)),
                                          C.byref(rp), s.ctypes.data_as(_dp), s2.ctypes.data_as(_dp)))
-        return s, s2, stats
+        return (s, s2, stats, led) if ledger else (s, s2, stats)
 
     def beamlets_sparse(self, pid, src, nhist, nbatch, rel_thr, seed=(97, 33), hist0=0, cap=None, nsplit=1):
         """matRad Dij as CSC arrays (jcs, irs, vals) plus the run statistics."""

@@ -323,6 +323,7 @@ static inline double pwlf_e(int idx, double l, const double *c1, const double *c
 /* rayleigh (src/ompmc.c:1102-1145): medium-0 form-factor tables, start bin 0 */
 static void ora_rayleigh(ora_ctx *c, int imed, double eig, double gle, int lgle)
 {
+    c->cnt.n_rayleigh++;
     const omc_tables *T = c->t;
     double pmax = pwlf_e(imed * MXGE + lgle, gle, T->pmax1, T->pmax0);
     double xmax = HC_INVERSE * eig;
@@ -361,6 +362,7 @@ static double pair_rej(const omc_tables *T, int imed, double xi, double esedei, 
 /* pair (src/ompmc.c:1429-1667) */
 static void ora_pair(ora_ctx *c, int imed)
 {
+    c->cnt.n_pair++;
     const omc_tables *T = c->t;
     int np = c->np;
     double eig = c->S[np].e;
@@ -469,6 +471,7 @@ static void ora_pair(ora_ctx *c, int imed)
 /* compton (src/ompmc.c:1670-1783), free-electron Klein-Nishina */
 static void ora_compton(ora_ctx *c)
 {
+    c->cnt.n_compton++;
     int np = c->np;
     double eig = c->S[np].e;
     double ko = c->S[np].e / RM;
@@ -534,6 +537,7 @@ static void ora_compton(ora_ctx *c)
 /* photo (src/ompmc.c:1786-1846): no binding energy, Sauter-type angle */
 static void ora_photo(ora_ctx *c)
 {
+    c->cnt.n_photo++;
     int np = c->np;
     c->npold = np;
     c->S[np].e += RM;
@@ -636,6 +640,7 @@ start_mfp_loop:
         imed = R->med[irl];
 
         do {
+            c->cnt.n_photon_steps++;
             if (imed != -1) {
                 lgle = pwlf_i(imed, gle, T->ge1, T->ge0) - 1;
                 gmfpr0 = pwlf_e(imed * MXGE + lgle, gle, T->gmfp1, T->gmfp0);
@@ -1116,6 +1121,7 @@ static void roulette_photons(ora_ctx *c)
 /* rannih (src/ompmc.c:4111-4167): annihilation at rest */
 static void ora_rannih(ora_ctx *c)
 {
+    c->cnt.n_rannih++;
     int np = c->np;
     c->npold = np;
     double rnno = RND();
@@ -1146,6 +1152,7 @@ static void ora_rannih(ora_ctx *c)
 /* brems (src/ompmc.c:4170-4356): BH energy sampling, KM-2BS angle */
 static void ora_brems(ora_ctx *c)
 {
+    c->cnt.n_brems++;
     const omc_tables *T = c->t;
     ora_particle *S = c->S;
     int np = c->np;
@@ -1247,6 +1254,7 @@ static void ora_brems(ora_ctx *c)
 /* moller (src/ompmc.c:4359-4435) */
 static void ora_moller(ora_ctx *c)
 {
+    c->cnt.n_moller++;
     const omc_tables *T = c->t;
     ora_particle *S = c->S;
     int np = c->np;
@@ -1288,6 +1296,7 @@ static void ora_moller(ora_ctx *c)
 /* bhabha (src/ompmc.c:4438-4525) */
 static void ora_bhabha(ora_ctx *c)
 {
+    c->cnt.n_bhabha++;
     const omc_tables *T = c->t;
     ora_particle *S = c->S;
     int np = c->np;
@@ -1338,6 +1347,7 @@ static void ora_bhabha(ora_ctx *c)
 /* annih (src/ompmc.c:4528-4645): two-photon annihilation in flight */
 static void ora_annih(ora_ctx *c)
 {
+    c->cnt.n_annih++;
     ora_particle *S = c->S;
     int np = c->np;
     double avip = S[np].e + RM;
@@ -1472,6 +1482,7 @@ static void ora_electron(ora_ctx *c)
              * loop after one boundary-crossing random number */
             double total_de = 0.0, total_tstep = 0.0;
             int call_howfar, do_single = 0, called_msdist = 0;
+            c->cnt.n_electron_steps++;
             double tstep, tustep, ekef, ekei, elkei, tuss, range = 0.0, p2, beta2, etap;
             double tvstep = 0.0, de = 0.0;
             double xf = 0, yf = 0, zf = 0, uf = 0, vf = 0, wf = 0;
@@ -1544,6 +1555,7 @@ static void ora_electron(ora_ctx *c)
                     called_msdist = 1;
                     de = ora_compute_eloss(c, imed, iq, rhof, tustep, range, eke, elke, lelke);
                     tvstep = tustep;
+                    c->cnt.n_msdist++;
                     ustep = msdist_core(c, imed, iq, rhof, de, tustep, eke, S[np].x, S[np].y, S[np].z,
                                         S[np].u, S[np].v, S[np].w, &xf, &yf, &zf, &uf, &vf, &wf);
                 } else {
@@ -1569,7 +1581,10 @@ static void ora_electron(ora_ctx *c)
             }
 
             int irold = S[np].ir, irnew = S[np].ir, idisc = 0;
-            if (call_howfar) ora_howfar(c, &S[np], &idisc, &irnew, &ustep);
+            if (call_howfar) {
+                c->cnt.n_howfar_charged++;
+                ora_howfar(c, &S[np], &idisc, &irnew, &ustep);
+            }
             if (idisc > 0) {
                 edep = (iq > 0) ? S[np].e + RM : S[np].e - RM;
                 ausgab(c, edep);
@@ -1621,6 +1636,7 @@ static void ora_electron(ora_ctx *c)
                     if (iq < 0) etap = pwlf_e(MXEKE * imed + lelkems, elkems, T->etae_ms1, T->etae_ms0);
                     else etap = pwlf_e(MXEKE * imed + lelkems, elkems, T->etap_ms1, T->etap_ms0);
                     chia2 *= etap;
+                    c->cnt.n_sscat++;
                     sscat(c, imed, qel, chia2, elkems, beta2, &costhe, &sinthe);
                 } else {
                     sinthe = 0.0f;
@@ -1674,6 +1690,7 @@ static void ora_electron(ora_ctx *c)
         }
         sigratio = sigf / sig0;
         rfict = RND();
+        if (rfict >= sigratio) c->cnt.n_fict++;
     } while (rfict >= sigratio);
 
     int ix = imed * MXEKE + lelke;
@@ -1930,7 +1947,8 @@ int ora_run_dosxyz_parallel(const omc_tables *t, const omc_regions *r, const omc
     double *batch = calloc((size_t)nbatch * nreg, sizeof(double));
     if (!batch) return OMC_ENOMEM;
     int rc = OMC_OK;
-    ora_counters tot = { 0, 0, 0, 0, 0 };
+    ora_counters tot;
+    memset(&tot, 0, sizeof tot);
 #ifdef _OPENMP
     if (nthreads < 1) nthreads = omp_get_num_procs();
 #pragma omp parallel num_threads(nthreads)
@@ -1955,11 +1973,12 @@ int ora_run_dosxyz_parallel(const omc_tables *t, const omc_regions *r, const omc
         {
             for (size_t i = 0; i < (size_t)nbatch * nreg; ++i) batch[i] += mine[i];
             if (c->err) rc = OMC_EOVERFLOW;
-            tot.n_hist += c->cnt.n_hist;
-            tot.n_dep += c->cnt.n_dep;
-            tot.n_cross_photon += c->cnt.n_cross_photon;
-            tot.n_cross_charged += c->cnt.n_cross_charged;
-            tot.n_draws += c->cnt.n_draws;
+            {
+                /* every field is an int64_t count */
+                int64_t *a = (int64_t *)&tot;
+                const int64_t *b = (const int64_t *)&c->cnt;
+                for (size_t k = 0; k < sizeof tot / sizeof(int64_t); ++k) a[k] += b[k];
+            }
         }
         free(mine);
         ora_destroy(c);

@@ -47,6 +47,15 @@ typedef struct {
     int64_t n_cross_photon;   /* photon region changes */
     int64_t n_cross_charged;  /* charged-particle region changes */
     int64_t n_draws;          /* random numbers consumed */
+    /* physics events, counted where the reference calls each routine (the
+     * device engine counts the same events in the same places) */
+    int64_t n_compton, n_photo, n_pair, n_rayleigh;
+    int64_t n_brems, n_moller, n_bhabha, n_annih, n_rannih;
+    int64_t n_fict;           /* discrete events rejected by sigf/sig0 */
+    int64_t n_msdist, n_sscat;
+    int64_t n_howfar_charged; /* howfar calls in electron() */
+    int64_t n_photon_steps;   /* iterations of photon()'s transport loop */
+    int64_t n_electron_steps; /* iterations of electron()'s ustep loop */
 } ora_counters;
 
 typedef struct ora_ctx ora_ctx;

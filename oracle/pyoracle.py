@@ -34,8 +34,11 @@ class Particle(C.Structure):
 
 
 class Counters(C.Structure):
-    _fields_ = [("n_hist", C.c_int64), ("n_dep", C.c_int64), ("n_cross_photon", C.c_int64),
-                ("n_cross_charged", C.c_int64), ("n_draws", C.c_int64)]
+    _fields_ = [(n, C.c_int64) for n in (
+        "n_hist", "n_dep", "n_cross_photon", "n_cross_charged", "n_draws",
+        "n_compton", "n_photo", "n_pair", "n_rayleigh", "n_brems", "n_moller", "n_bhabha", "n_annih",
+        "n_rannih", "n_fict", "n_msdist", "n_sscat", "n_howfar_charged", "n_photon_steps",
+        "n_electron_steps")]
 
 
 _lib = None
@@ -105,6 +108,36 @@ def run_dosxyz_ranmar(prob: DosxyzProblem, nperbatch=None, nbatch=None):
         return s, s2, cnt
     finally:
         L.ora_destroy(ctx)
+
+
+def run_dosxyz_ranmar_runs(prob: DosxyzProblem, nruns, nperbatch, nbatch, threads=None):
+    """`nruns` independent serial RANMAR runs of the problem (the reference's
+    own generator and history loop), each with seeds (ixx, jxx + k), run on
+    up to `threads` host threads at once; their batches are pooled.  Returns
+    (sum, sum2, total batches) over all runs."""
+    import concurrent.futures as cf
+    import os
+    L = lib()
+    nreg = prob.reg.nreg
+    threads = threads or min(nruns, len(os.sched_getaffinity(0)), 16)
+
+    def one(k):
+        ctx = L.ora_create(C.byref(prob.tab), C.byref(prob.reg), C.byref(prob.ph), prob.nsplit, RANMAR)
+        try:
+            L.ora_seed_ranmar(ctx, prob.seeds[0], (prob.seeds[1] + k) % 30081)
+            s = np.zeros(nreg)
+            s2 = np.zeros(nreg)
+            rc = L.ora_run_dosxyz(ctx, C.byref(prob.src), nperbatch, nbatch, s.ctypes.data_as(_dp),
+                                  s2.ctypes.data_as(_dp))
+            if rc:
+                raise RuntimeError(f"oracle run failed {rc}")
+            return s, s2
+        finally:
+            L.ora_destroy(ctx)
+
+    with cf.ThreadPoolExecutor(max_workers=threads) as ex:
+        res = list(ex.map(one, range(nruns)))
+    return sum(r[0] for r in res), sum(r[1] for r in res), nruns * nbatch
 
 
 def run_dosxyz_philox(prob: DosxyzProblem, seed=(97, 33), nperbatch=None, nbatch=None, nthreads=0,

@@ -103,16 +103,20 @@ def reference_tables(media, work: Path):
 
 
 def main():
+    import argparse
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--cases", nargs="*", help="regenerate only these dose cases (no table sets)")
+    args = ap.parse_args()
     if not (REF / "ref_dump").exists():
         sys.exit("oracle/_ref not built (make -C oracle ref)")
-    for set_name, media in MEDIA_SETS.items():
+    for set_name, media in (MEDIA_SETS.items() if args.cases is None else ()):
         with tempfile.TemporaryDirectory() as td:
             arrays = reference_tables(media, Path(td))
             js = {"media": media, "source": "reference ref_dump (oracle/_ref)",
                   "tables": table_hashes(arrays, len(media))}
             (HERE / f"tables_{set_name}.json").write_text(json.dumps(js, indent=1))
             print("tables", set_name)
-    for case in CASES:
+    for case in (args.cases if args.cases else CASES):
         with tempfile.TemporaryDirectory() as td:
             stem = write_case(case, Path(td))
             r = subprocess.run([str(REF / "omc_dosxyz_ref"), "-i", str(stem), "-o", "ref"],
