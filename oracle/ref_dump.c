@@ -5,10 +5,15 @@
  *       runs the reference initialisation (initPhantom, initMediaData,
  *       initSource, initRegions) and writes every table the transport reads
  *       as raw little-endian arrays <outdir>/<name>.bin
- *   ref_dump vectors <inp-stem> <outdir> <tape-file> <n>
- *       calls individual reference transport functions on seeded states with
- *       the random-number stream replaced by a tape of 24-bit integers, and
- *       writes their outputs (per-function parity vectors)
+ *   ref_dump vectors <inp-stem> <outdir> <n>
+ *       calls individual reference functions on n generated states each
+ *       (a fixed xorshift sequence, so the vectors are reproducible) and
+ *       writes inputs and outputs side by side (per-function parity
+ *       vectors): howfar and hownear on the geometry's voxels (positions on
+ *       and between boundaries, axis-parallel and oblique directions, the
+ *       outside region), computeDrange and computeEloss at the arguments
+ *       electron() passes (ompmc.c:4832-4926, 4983), and the photon table
+ *       lookups pwlfInterval / pwlfEval (ompmc.c:1952-1969)
  *
  * The outputs are consumed by tests/golden/make_golden.py, which turns them
  * into hashes and small fixtures.
@@ -17,6 +22,8 @@
 #include "omc_utilities.h"
 #include "omc_random.h"
 
+#include <math.h>
+#include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -124,15 +131,125 @@ static int dump_tables(void)
     return 0;
 }
 
+/* ---- per-function vectors -------------------------------------------------- */
+static uint64_t g_rs = 0x9E3779B97F4A7C15ull;
+
+static double uni(void)
+{
+    g_rs ^= g_rs >> 12;
+    g_rs ^= g_rs << 25;
+    g_rs ^= g_rs >> 27;
+    return (double)((g_rs * 0x2545F4914F6CDD1Dull) >> 11) * (1.0 / 9007199254740992.0);
+}
+
+static int pick(int n) { int k = (int)(uni() * n); return k < n ? k : n - 1; }
+
+static FILE *vopen(const char *name)
+{
+    char path[1024];
+    snprintf(path, sizeof path, "%s/%s.bin", g_out, name);
+    FILE *fp = fopen(path, "wb");
+    if (!fp) { fprintf(stderr, "cannot write %s\n", path); exit(2); }
+    return fp;
+}
+
+/* one record = doubles (integers stored as doubles, exact below 2^53) */
+static void rec(FILE *fp, const double *v, int n) { fwrite(v, sizeof(double), (size_t)n, fp); }
+
+static int dump_vectors(int n)
+{
+    const int nx = refacc_isize(), ny = refacc_jsize(), nz = refacc_ksize();
+    const double *xb = refacc_bounds(0), *yb = refacc_bounds(1), *zb = refacc_bounds(2);
+    const int nmed = media.nmed;
+    initStack();
+    stack.np = 0;
+    /* howfar / hownear: record irl x y z u v w ustep_in | idisc irnew ustep_out tperp */
+    FILE *fg = vopen("vec_geometry");
+    for (int k = 0; k < n; ++k) {
+        int ix = pick(nx), iy = pick(ny), iz = pick(nz);
+        int irl = (k % 17 == 0) ? 0 : 1 + ix + iy * nx + iz * nx * ny;
+        double f[3];
+        for (int d = 0; d < 3; ++d) {
+            const int mode = pick(8);          /* 0: lower plane, 1: upper plane, else inside */
+            f[d] = mode == 0 ? 0.0 : mode == 1 ? 1.0 : uni();
+        }
+        double x = xb[ix] + f[0] * (xb[ix + 1] - xb[ix]);
+        double y = yb[iy] + f[1] * (yb[iy + 1] - yb[iy]);
+        double z = zb[iz] + f[2] * (zb[iz + 1] - zb[iz]);
+        double u, v, w;
+        if (k % 5 == 0) {                      /* axis-parallel, either sense */
+            const int a = pick(3);
+            const double sgn = uni() < 0.5 ? -1.0 : 1.0;
+            u = a == 0 ? sgn : 0.0; v = a == 1 ? sgn : 0.0; w = a == 2 ? sgn : 0.0;
+        } else {
+            const double ct = 2.0 * uni() - 1.0, ph = 6.283185307179586 * uni();
+            const double st = sqrt(1.0 - ct * ct);
+            u = st * cos(ph); v = st * sin(ph); w = ct;
+        }
+        const double dx = xb[ix + 1] - xb[ix];
+        const double ustep_in = (k % 7 == 0) ? 1.0E8 : 3.0 * dx * uni();
+        stack.ir[0] = irl;
+        stack.x[0] = x; stack.y[0] = y; stack.z[0] = z;
+        stack.u[0] = u; stack.v[0] = v; stack.w[0] = w;
+        int idisc = 0, irnew = irl;
+        double ustep = ustep_in;
+        howfar(&idisc, &irnew, &ustep);
+        const double tperp = hownear();
+        const double r[12] = { irl, x, y, z, u, v, w, ustep_in, idisc, irnew, ustep, tperp };
+        rec(fg, r, 12);
+    }
+    fclose(fg);
+    /* computeDrange at electron()'s range call and computeEloss at its CH
+     * step: imed iq lelke eke elke ekei elkei rhof tustep range | drange de */
+    FILE *fe = vopen("vec_csda");
+    for (int k = 0; k < n; ++k) {
+        const int imed = pick(nmed), iq = uni() < 0.5 ? -1 : 1, qel = (1 + iq) / 2;
+        const double lo = log(pegs_data.ae[imed] - RM), hi = log(pegs_data.ue[imed] - RM);
+        const double eke = exp(lo + (hi - lo) * (0.001 + 0.998 * uni()));
+        const double elke = log(eke);
+        const int lelke = pwlfInterval(imed, elke, electron_data.eke1, electron_data.eke0) - 1;
+        const double ekei = electron_data.e_array[imed * MXEKE + lelke];
+        const double elkei = (lelke + 1 - electron_data.eke0[imed]) / electron_data.eke1[imed];
+        const double dr = computeDrange(imed, iq, lelke, eke, ekei, elke, elkei);
+        const double rhof = 0.2 + 1.8 * uni();
+        double range = dr + electron_data.range_ep[qel * nmed * MXEKE + imed * MXEKE + lelke];
+        range /= rhof;
+        const double tustep = range * (k % 9 == 0 ? 1.0 : uni());
+        const double de = computeEloss(imed, iq, 1, rhof, tustep, range, eke, elke, lelke);
+        const double r[12] = { imed, iq, lelke, eke, elke, ekei, elkei, rhof, tustep, range, dr, de };
+        rec(fe, r, 12);
+    }
+    fclose(fe);
+    /* photon table lookups: imed e | lgle gmfp cohfac gbr1 gbr2 */
+    FILE *fp = vopen("vec_pwlf");
+    for (int k = 0; k < n; ++k) {
+        const int imed = pick(nmed);
+        const double lo = log(pegs_data.ap[imed]), hi = log(pegs_data.up[imed]);
+        const double e = exp(lo + (hi - lo) * (k % 11 == 0 ? 1e-9 : uni()));
+        const double gle = log(e);
+        const int lgle = pwlfInterval(imed, gle, photon_data.ge1, photon_data.ge0) - 1;
+        const int ix = imed * MXGE + lgle;
+        const double r[7] = { imed, e, lgle, pwlfEval(ix, gle, photon_data.gmfp1, photon_data.gmfp0),
+                              pwlfEval(ix, gle, photon_data.cohe1, photon_data.cohe0),
+                              pwlfEval(ix, gle, photon_data.gbr11, photon_data.gbr10),
+                              pwlfEval(ix, gle, photon_data.gbr21, photon_data.gbr20) };
+        rec(fp, r, 7);
+    }
+    fclose(fp);
+    return 0;
+}
+
 int main(int argc, char **argv)
 {
     if (argc < 4) {
-        fprintf(stderr, "usage: ref_dump tables <inp-stem> <outdir>\n");
+        fprintf(stderr, "usage: ref_dump tables <inp-stem> <outdir>\n"
+                        "       ref_dump vectors <inp-stem> <outdir> <n>\n");
         return 2;
     }
     g_out = argv[3];
     init_all(argv[2]);
     if (!strcmp(argv[1], "tables")) return dump_tables();
+    if (!strcmp(argv[1], "vectors")) return dump_vectors(argc > 4 ? atoi(argv[4]) : 1000);
     fprintf(stderr, "unknown mode %s\n", argv[1]);
     return 2;
 }

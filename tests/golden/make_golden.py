@@ -5,6 +5,10 @@ Run in a container where the ompMC sources are present (make -C oracle ref).
 Writes, per case in cases.py:
   ref_dose_<case>.json   SHA-256 of the reference omc_dosxyz (serial build)
                          .3ddose output plus summary numbers
+per-function vectors of the reference on the slab case:
+  vectors_slab6MV.npz    howfar / hownear, computeDrange / computeEloss and
+                         the photon table lookups, inputs and outputs
+                         (ref_dump vectors)
 and per media set:
   tables_<set>.json      SHA-256 of every table the transport reads, as
                          dumped from the reference's globals by ref_dump,
@@ -116,6 +120,19 @@ def main():
                   "tables": table_hashes(arrays, len(media))}
             (HERE / f"tables_{set_name}.json").write_text(json.dumps(js, indent=1))
             print("tables", set_name)
+    if args.cases is None:
+        # per-function vectors of the reference (ref_dump vectors) on the
+        # slab problem's geometry and media
+        with tempfile.TemporaryDirectory() as td:
+            stem = write_case("slab6MV", Path(td))
+            out = Path(td) / "vec"
+            out.mkdir()
+            subprocess.run([str(REF / "ref_dump"), "vectors", str(stem), str(out), "600"], check=True,
+                           capture_output=True)
+            arrs = {n: np.fromfile(out / f"vec_{n}.bin", dtype="<f8").reshape(-1, w)
+                    for n, w in (("geometry", 12), ("csda", 12), ("pwlf", 7))}
+            np.savez_compressed(HERE / "vectors_slab6MV.npz", **arrs)
+            print("vectors", {k: v.shape for k, v in arrs.items()})
     for case in (args.cases if args.cases else CASES):
         with tempfile.TemporaryDirectory() as td:
             stem = write_case(case, Path(td))
