@@ -250,6 +250,26 @@ int omc_write_3ddose(const char *path, const omc_phantom *p,
 int omc_read_3ddose(const char *path, int *isize, int *jsize, int *ksize,
                     double **dose, double **unc);
 
+/* Equal-accuracy comparison of two dose distributions with their relative
+ * uncertainties (the .3ddose contents, omc_dosxyz.c:801-886): per voxel
+ * z = (d_a - d_b) / sqrt((u_a d_a)^2 + (u_b d_b)^2) over the voxels whose mean
+ * of the two doses exceeds rel_thr times its maximum; pass = at least 95% of
+ * them within |z| <= 2 and the integral (dose times mass if masses are given,
+ * else dose) within 0.5%.  No counterpart in the reference; it is the check
+ * the GPU results are held to against the reference's. */
+typedef struct {
+    int64_t voxels;               /* voxels compared */
+    double  frac_within_2sigma;
+    double  z_mean, z_rms, z_max_abs;
+    double  integral_ratio;       /* a / b */
+    int     pass;
+} omc_dose_comparison;
+int omc_compare_doses(int64_t nvox, const double *dose_a, const double *unc_a, const double *dose_b,
+                      const double *unc_b, const double *mass, double rel_thr, omc_dose_comparison *out);
+/* The same on two .3ddose files; `phantom` (may be NULL) gives the masses. */
+int omc_compare_3ddose(const char *path_a, const char *path_b, const omc_phantom *phantom, double rel_thr,
+                       omc_dose_comparison *out);
+
 /* matRad sparse column: voxels with dose > thr*max (omc_matrad.c:1420-1479).
  * Writes row indices (0-based voxel) and values; returns the count. */
 int64_t omc_sparse_column(const double *dose, int64_t nvox, double rel_thr,

@@ -113,6 +113,23 @@ class SourceBeamlets(C.Structure):
                 ("side1", _dp), ("side2", _dp)]
 
 
+class DoseComparison(C.Structure):
+    _fields_ = [("voxels", C.c_int64), ("frac_within_2sigma", C.c_double), ("z_mean", C.c_double),
+                ("z_rms", C.c_double), ("z_max_abs", C.c_double), ("integral_ratio", C.c_double),
+                ("pass_", C.c_int)]
+
+    def as_dict(self):
+        return {("pass" if k == "pass_" else k): getattr(self, k) for k, _ in self._fields_}
+
+
+def compare_3ddose(path_a, path_b, phantom=None, rel_thr=0.1):
+    """omc_compare_3ddose: equal-accuracy statistics of two .3ddose files."""
+    r = DoseComparison()
+    check(host().omc_compare_3ddose(str(path_a).encode(), str(path_b).encode(),
+                                    C.byref(phantom) if phantom is not None else None, rel_thr, C.byref(r)))
+    return r.as_dict()
+
+
 def beamlet_source(spec, ibeam, src, corner, side1, side2, charge=0):
     """omc_source_beamlets over numpy arrays; the arrays are kept alive on
     the returned structure."""
@@ -167,6 +184,10 @@ def host():
                                                 C.c_double, C.c_double, C.c_double, C.c_double, C.c_int]
         lib.omc_batch_results.argtypes = [C.POINTER(Phantom), _dp, _dp, C.c_int, C.c_double, C.c_int, _dp, _dp]
         lib.omc_write_3ddose.argtypes = [C.c_char_p, C.POINTER(Phantom), _dp, _dp]
+        lib.omc_compare_doses.argtypes = [C.c_int64, _dp, _dp, _dp, _dp, _dp, C.c_double,
+                                          C.POINTER(DoseComparison)]
+        lib.omc_compare_3ddose.argtypes = [C.c_char_p, C.c_char_p, C.POINTER(Phantom), C.c_double,
+                                           C.POINTER(DoseComparison)]
         lib.omc_sparse_column.argtypes = [_dp, C.c_int64, C.c_double, C.POINTER(C.c_int64), _dp, C.c_int64]
         lib.omc_sparse_column.restype = C.c_int64
         lib.omc_inp_parse_text.argtypes = [C.c_void_p, C.c_char_p]

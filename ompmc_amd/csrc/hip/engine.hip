@@ -97,6 +97,9 @@ constexpr int64_t drain_min_default(int cus, int precision)
 // the FP32 slab ran 133 / 140 / 144 / 145 M histories/s at fetch thresholds
 // of 16 / 4 / 8 / 1 lanes (FP64 and the beamlet batch within noise).
 constexpr uint32_t DEFAULT_EVENT_DIV = 2;
+// Condensed-history steps of the charged-particle kernel wait the same way
+// for half the live lanes of their wavefront.
+constexpr uint32_t DEFAULT_MSDIST_DIV = 2;
 constexpr uint32_t DEFAULT_FETCH_MIN = 1;
 
 struct PhantomSlot {
@@ -159,6 +162,7 @@ struct Work {
     std::vector<double> cdf;      // [2*OMC_INVDIM] inverse CDF
     int64_t nhist = 0, nperbatch = 0;
     int nbatch = 1;
+    int slots = 1;                // tally grids: nbatch, or 1 when only the mean is wanted
     int64_t nreg = 0;
     bool ledger = false;
     int64_t ledger_off = 0;       // first history of this work in the run's ledger
@@ -213,7 +217,7 @@ struct omc_dev_ctx {
     uint32_t yield_trips = 0;
     uint32_t drain_lanes = 0, drain_trips = 0;
     int64_t drain_min = 0;            // smallest launch input that drains
-    uint32_t event_div = 0, fetch_min = 0;
+    uint32_t event_div = 0, fetch_min = 0, msdist_div = 0;
     int cus = 256;
     std::chrono::steady_clock::time_point t0;
 
@@ -674,7 +678,7 @@ static void end_work(omc_dev_ctx *c, Work *w, int status)
 // Bytes a work needs on admission.
 static size_t work_bytes(const Work *w)
 {
-    return sizeof(double) * (size_t)w->nreg * w->nbatch + 2 * OMC_INVDIM * sizeof(double) +
+    return sizeof(double) * (size_t)w->nreg * w->slots + 2 * OMC_INVDIM * sizeof(double) +
            (w->ledger ? sizeof(double) * 3 * (size_t)w->nhist : 0);
 }
 
@@ -705,7 +709,7 @@ static int admit(omc_dev_ctx *c)
         }
         JobRec &J = c->jobs[slot];
         J = JobRec();
-        J.tally_bytes = sizeof(double) * (size_t)w->nreg * w->nbatch;
+        J.tally_bytes = sizeof(double) * (size_t)w->nreg * w->slots;
         int rc = pool_alloc(c, (void **)&J.tally, J.tally_bytes);
         if (!rc) rc = dalloc(c, (void **)&J.cdf, 2 * OMC_INVDIM * sizeof(double));
         if (!rc && w->ledger) {
@@ -729,14 +733,15 @@ static int admit(omc_dev_ctx *c)
         const int64_t npipes = (int64_t)c->pipes.size();
         J.quantum = w->ticket >= 0 ? std::max<int64_t>(4096, c->chunk / 8)
                                    : std::max<int64_t>(65536, (w->nhist + npipes - 1) / npipes);
-        launch_zero_f64(J.tally, w->nreg * w->nbatch, c->aux);
+        launch_zero_f64(J.tally, w->nreg * w->slots, c->aux);
         if (J.ledger) launch_zero_f64(J.ledger, 3 * w->nhist, c->aux);
         HIPCHK(hipMemcpyAsync(J.cdf, w->cdf.data(), 2 * OMC_INVDIM * sizeof(double), hipMemcpyHostToDevice, c->aux));
         DevJob d = w->tmpl;
         d.phantom = w->phantom;
         d.cdf1 = J.cdf;
         d.cdf2 = J.cdf + OMC_INVDIM;
-        d.nperbatch = w->nperbatch;
+        // one grid per batch; a job that keeps one grid puts every history in it
+        d.nperbatch = w->slots == w->nbatch ? w->nperbatch : w->nhist;
         d.tally = J.tally;
         d.ledger = J.ledger;
         HIPCHK(hipMemcpyAsync(c->d_jobs + slot, &d, sizeof d, hipMemcpyHostToDevice, c->aux));
@@ -781,7 +786,7 @@ static int finish_fold(omc_dev_ctx *c, int j)
     if (k && k->sparse) {
         const PhantomSlot &P = c->ph[w->phantom];
         const double inc = (double)(w->nperbatch * (int64_t)w->nbatch);
-        launch_beamlet_column(J.tally, w->nbatch, P.isize, P.jsize, P.ksize, (const double *)P.xb.p,
+        launch_beamlet_column(J.tally, w->slots, w->nbatch, P.isize, P.jsize, P.ksize, (const double *)P.xb.p,
                               (const double *)P.yb.p, (const double *)P.zb.p, (const double *)P.rho.p, inc,
                               k->rel_thr, c->sp, s);
         const int nblk = sparse_blocks((int64_t)P.isize * P.jsize * P.ksize);
@@ -969,6 +974,7 @@ template <typename R> static int launch_generation(omc_dev_ctx *c, int pi)
     A.drain_trips = c->drain_trips;
     A.drain_min = (uint32_t)std::min<int64_t>(c->drain_min, UINT32_MAX);
     A.event_div = c->event_div;
+    A.msdist_div = c->msdist_div;
     A.fetch_min = c->fetch_min;
     A.woodcock = c->woodcock ? 1u : 0u;
     const int max_grid = c->cus * 8;
@@ -1290,6 +1296,7 @@ int omc_dev_create(omc_dev_ctx **out, const omc_dev_opts *opts)
     c->drain_trips = o.drain_trips > 0 ? (uint32_t)o.drain_trips : DEFAULT_DRAIN_TRIPS;
     c->event_div = o.event_div > 0 ? (uint32_t)o.event_div : DEFAULT_EVENT_DIV;
     c->fetch_min = o.fetch_min > 0 ? (uint32_t)std::min(o.fetch_min, 64) : DEFAULT_FETCH_MIN;
+    c->msdist_div = o.msdist_div > 0 ? (uint32_t)std::min(o.msdist_div, 64) : DEFAULT_MSDIST_DIV;
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, o.device) == hipSuccess) c->cus = prop.multiProcessorCount;
     c->drain_min = o.drain_min > 0 ? o.drain_min : o.drain_min < 0 ? 0 : drain_min_default(c->cus, c->precision);
@@ -1441,6 +1448,7 @@ static int set_histories(Work &w, int64_t nperbatch, int nbatch)
     if (nperbatch * (int64_t)nbatch > 0xffffffffLL) return fail(OMC_ERANGE, "more than 2^32 histories in one job");
     w.nperbatch = nperbatch;
     w.nbatch = nbatch;
+    w.slots = nbatch;
     w.nhist = nperbatch * nbatch;
     return OMC_OK;
 }
@@ -1474,13 +1482,13 @@ namespace {
 struct TuneSetting {
     int pipes;
     int64_t chunk;
-    int yield_trips, drain_lanes, drain_trips, event_div, fetch_min;
+    int yield_trips, drain_lanes, drain_trips, event_div, fetch_min, msdist_div;
 };
 
 TuneSetting current_setting(const omc_dev_ctx *c)
 {
     return { (int)c->pipes.size(), c->chunk, (int)c->yield_trips, (int)c->drain_lanes, (int)c->drain_trips,
-             (int)c->event_div, (int)c->fetch_min };
+             (int)c->event_div, (int)c->fetch_min, (int)c->msdist_div };
 }
 
 int apply_setting(omc_dev_ctx *c, const TuneSetting &t)
@@ -1491,6 +1499,7 @@ int apply_setting(omc_dev_ctx *c, const TuneSetting &t)
     c->drain_trips = (uint32_t)t.drain_trips;
     c->event_div = (uint32_t)t.event_div;
     c->fetch_min = (uint32_t)t.fetch_min;
+    c->msdist_div = (uint32_t)t.msdist_div;
     return OMC_OK;
 }
 
@@ -1519,7 +1528,7 @@ static int tune_impl(omc_dev_ctx *c, const std::function<int()> &run, int64_t hi
         }
         return OMC_OK;
     };
-    static const std::vector<int64_t> axes[7] = {
+    static const std::vector<int64_t> axes[8] = {
         { 2, 4, 6, 8 },                              // pipelines
         { 1 << 20, 1 << 21, 1 << 22, 1 << 23 },      // live particles per pipeline
         { 1024, 4096, 16384 },                       // yield budget (trips)
@@ -1527,15 +1536,16 @@ static int tune_impl(omc_dev_ctx *c, const std::function<int()> &run, int64_t hi
         { 8, 16, 32 },                               // drain trips
         { 1, 2, 4 },                                 // photon event threshold divisor
         { 1, 4, 8, 16 },                             // fetch threshold (lanes)
+        { 1, 2, 3, 4 },                              // multiple-scattering threshold divisor
     };
     TuneSetting bst = current_setting(c);
     double bv = 0;
     int rc = measure(bst, bv);
-    for (int a = 0; a < 7 && !rc; ++a) {
+    for (int a = 0; a < 8 && !rc; ++a) {
         for (int64_t val : axes[a]) {
             TuneSetting t = bst;
-            int *fields[7] = { &t.pipes, nullptr, &t.yield_trips, &t.drain_lanes, &t.drain_trips, &t.event_div,
-                               &t.fetch_min };
+            int *fields[8] = { &t.pipes, nullptr, &t.yield_trips, &t.drain_lanes, &t.drain_trips, &t.event_div,
+                               &t.fetch_min, &t.msdist_div };
             if (a == 1) {
                 if (t.chunk == val) continue;
                 t.chunk = val;
@@ -1569,6 +1579,7 @@ static int tune_impl(omc_dev_ctx *c, const std::function<int()> &run, int64_t hi
     best->woodcock = c->woodcock ? 1 : -1;
     best->event_div = (int)c->event_div;
     best->fetch_min = (int)c->fetch_min;
+    best->msdist_div = (int)c->msdist_div;
     *hps = bv;
     return OMC_OK;
 }
@@ -1641,7 +1652,10 @@ int omc_dev_beamlets_sparse(omc_dev_ctx *c, const omc_source_beamlets *src, cons
     if (src) {
         std::vector<Work> works;
         OMC_TRY_DEV(beamlet_works(c, src, nhist, rp, works));
-        for (auto &w : works) w.ledger = false;       // the Dij call returns no ledger
+        // the Dij uses the mean dose only (accumulateResults' uncertainty is
+        // not part of it, omc_matrad.c:1416-1479): one FP64 grid per beamlet
+        // instead of one per batch, and no ledger
+        for (auto &w : works) { w.ledger = false; w.slots = 1; }
         HIPCHK(hipSetDevice(c->device));
         RunSink k;
         k.beamlets = true;

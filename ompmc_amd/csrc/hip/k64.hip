@@ -94,7 +94,8 @@ __global__ void k_bl_init(unsigned long long *maxkey)
 // accumulateResults with iout = 1 (omc_matrad.c:839-918) for one beamlet:
 // batch sums in batch order, mean over batches, conversion to Gy per
 // incident history, zero in voxels with density < 0.044.
-__global__ void __launch_bounds__(256) k_bl_dose(const double *tally, int nbatch, int isize, int jsize, int ksize,
+__global__ void __launch_bounds__(256) k_bl_dose(const double *tally, int nslots, int nbatch, int isize, int jsize,
+                                                 int ksize,
                                                  const double *xb, const double *yb, const double *zb,
                                                  const double *rho, double inc_fluence, double *dose,
                                                  unsigned long long *maxkey)
@@ -105,7 +106,7 @@ __global__ void __launch_bounds__(256) k_bl_dose(const double *tally, int nbatch
     unsigned long long key = dkey(0.0);
     if (v < nvox) {
         double e = 0.0;
-        for (int b = 0; b < nbatch; ++b) e += tally[(size_t)b * nreg + v + 1];
+        for (int b = 0; b < nslots; ++b) e += tally[(size_t)b * nreg + v + 1];
         e /= (double)nbatch;
         const int ix = (int)(v % isize), iy = (int)((v / isize) % jsize), iz = (int)(v / ((int64_t)isize * jsize));
         double mass = (xb[ix + 1] - xb[ix]) * (yb[iy + 1] - yb[iy]) * (zb[iz + 1] - zb[iz]);
@@ -193,14 +194,14 @@ __global__ void __launch_bounds__(256) k_bl_emit(const double *dose, int64_t nvo
 
 int sparse_blocks(int64_t nvox) { return (int)((nvox + 255) / 256); }
 
-void launch_beamlet_column(const double *tally, int nbatch, int isize, int jsize, int ksize, const double *xb,
+void launch_beamlet_column(const double *tally, int nslots, int nbatch, int isize, int jsize, int ksize, const double *xb,
                            const double *yb, const double *zb, const double *rho, double inc_fluence,
                            double rel_thr, const SparseScratch &S, hipStream_t st)
 {
     const int64_t nvox = (int64_t)isize * jsize * ksize;
     const int nblk = sparse_blocks(nvox);
     hipLaunchKernelGGL(k_bl_init, dim3(1), dim3(64), 0, st, S.maxkey);
-    hipLaunchKernelGGL(k_bl_dose, dim3(nblk), dim3(256), 0, st, tally, nbatch, isize, jsize, ksize, xb, yb, zb,
+    hipLaunchKernelGGL(k_bl_dose, dim3(nblk), dim3(256), 0, st, tally, nslots, nbatch, isize, jsize, ksize, xb, yb, zb,
                        rho, inc_fluence, S.dose, S.maxkey);
     hipLaunchKernelGGL(k_bl_count, dim3(nblk), dim3(256), 0, st, S.dose, nvox, S.maxkey, rel_thr, S.blk);
     hipLaunchKernelGGL(k_bl_scan, dim3(1), dim3(1024), 0, st, S.blk, nblk);

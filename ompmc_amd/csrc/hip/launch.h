@@ -33,8 +33,10 @@ void launch_job_flags(const uint32_t *meta_p, const uint32_t *count_p, const uin
                       const uint32_t *count_e, uint32_t cap, uint32_t *flags, int njobs, hipStream_t st);
 
 // matRad beamlet column on the device (omc_matrad.c:839-918 with iout = 1,
-// then 1420-1479): dose[v] from the batch tallies, its maximum, and the
-// compaction of the voxels above rel_thr * max into rows/vals in voxel order.
+// then 1420-1479): dose[v] from the tally grids (nslots of them: one per
+// batch, or a single one holding every batch) summed and divided by nbatch,
+// its maximum, and the compaction of the voxels above rel_thr * max into
+// rows/vals in voxel order.
 struct SparseScratch {
     double *dose;                 // [nvox]
     unsigned long long *maxkey;   // [1] order-preserving key of the maximum
@@ -43,7 +45,7 @@ struct SparseScratch {
     double *vals;                 // [nvox]
 };
 int sparse_blocks(int64_t nvox);
-void launch_beamlet_column(const double *tally, int nbatch, int isize, int jsize, int ksize, const double *xb,
+void launch_beamlet_column(const double *tally, int nslots, int nbatch, int isize, int jsize, int ksize, const double *xb,
                            const double *yb, const double *zb, const double *rho, double inc_fluence,
                            double rel_thr, const SparseScratch &S, hipStream_t st);
 
