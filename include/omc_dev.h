@@ -35,6 +35,13 @@ typedef struct omc_dev_ctx omc_dev_ctx;
 
 #define OMC_MAX_NSPLIT 1023   /* photon splitting factor the device supports */
 
+/* Entries per history of the optional per-history ledger: energy deposited
+ * in the phantom, energy deposited in region 0 (outside), energy carried
+ * out by photons, and a path checksum (the sum over deposits in the phantom
+ * of energy times region index), which differs as soon as a history puts
+ * any energy into a different voxel. */
+#define OMC_LEDGER 4
+
 enum { OMC_PREC_FP64 = 0,   /* all transport in FP64 (plain GPU kernel) */
        OMC_PREC_FP32 = 1 }; /* FP32 transport, FP64 tallies and CSDA inversion */
 
@@ -152,9 +159,8 @@ typedef struct {
     int64_t  hist0;             /* index of the first history (stream id) */
     int64_t  nperbatch;         /* histories per statistical batch */
     int      nbatch;            /* number of batches */
-    double  *ledger;            /* optional [3*nperbatch*nbatch]: per history
-                                   energy deposited in the phantom, deposited
-                                   in region 0, carried out by photons */
+    double  *ledger;            /* optional [OMC_LEDGER*nperbatch*nbatch]:
+                                   per history the entries OMC_LEDGER names */
     omc_dev_stats *stats;       /* optional */
 } omc_dev_run_params;
 
@@ -174,7 +180,7 @@ int  omc_dev_run_dosxyz(omc_dev_ctx *ctx, const omc_source_dosxyz *src,
  * starting at rp->hist0.  rp->nperbatch is not used.
  * sum [b*nreg + i] and sum2[b*nreg + i] are increased by the batch sums,
  * as omc_dev_run_dosxyz does for one field.  rp->ledger, if set, receives
- * 3 entries per history of the run, beamlet after beamlet. */
+ * OMC_LEDGER entries per history of the run, beamlet after beamlet. */
 int  omc_dev_run_beamlets(omc_dev_ctx *ctx, const omc_source_beamlets *src,
                           const int64_t *nhist, const omc_dev_run_params *rp,
                           double *sum, double *sum2);
@@ -214,6 +220,16 @@ int  omc_dev_tune(omc_dev_ctx *ctx, const omc_source_dosxyz *src, const omc_dev_
  * batch itself or a subset) under each setting; no dose is copied back. */
 int  omc_dev_tune_beamlets(omc_dev_ctx *ctx, const omc_source_beamlets *src, const int64_t *nhist,
                            const omc_dev_run_params *rp, omc_dev_opts *best, double *hps);
+
+/* Diagnostic, no counterpart in the reference: evaluates the elementary
+ * functions the transport kernels use -- fn 0 log, 1 exp, 2 sqrt, 3 sin,
+ * 4 cos, 5 pow(x, y) -- on the device in the given precision, exactly as
+ * the kernels of that precision compile them, for n arguments (y only for
+ * pow; inputs are rounded to FP32 for OMC_PREC_FP32).  Comparing the results
+ * with the host C library names the functions through which a device
+ * history can depart from the CPU one.  Allocates its own device memory. */
+int  omc_dev_math_eval(int device, int precision, int fn, int64_t n, const double *x, const double *y,
+                       double *out);
 
 /* ---- serving: continuous batching of dose requests ----------------------- */
 typedef struct {

@@ -6,7 +6,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
-#include "omc_host.h"
+#include "omc_dev.h"
 
 namespace omc {
 

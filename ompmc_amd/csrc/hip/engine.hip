@@ -679,7 +679,7 @@ static void end_work(omc_dev_ctx *c, Work *w, int status)
 static size_t work_bytes(const Work *w)
 {
     return sizeof(double) * (size_t)w->nreg * w->slots + 2 * OMC_INVDIM * sizeof(double) +
-           (w->ledger ? sizeof(double) * 3 * (size_t)w->nhist : 0);
+           (w->ledger ? sizeof(double) * OMC_LEDGER * (size_t)w->nhist : 0);
 }
 
 static int any_running(const omc_dev_ctx *c)
@@ -713,7 +713,7 @@ static int admit(omc_dev_ctx *c)
         int rc = pool_alloc(c, (void **)&J.tally, J.tally_bytes);
         if (!rc) rc = dalloc(c, (void **)&J.cdf, 2 * OMC_INVDIM * sizeof(double));
         if (!rc && w->ledger) {
-            J.ledger_bytes = sizeof(double) * 3 * (size_t)w->nhist;
+            J.ledger_bytes = sizeof(double) * OMC_LEDGER * (size_t)w->nhist;
             rc = dalloc(c, (void **)&J.ledger, J.ledger_bytes);
         }
         if (rc) {
@@ -734,7 +734,7 @@ static int admit(omc_dev_ctx *c)
         J.quantum = w->ticket >= 0 ? std::max<int64_t>(4096, c->chunk / 8)
                                    : std::max<int64_t>(65536, (w->nhist + npipes - 1) / npipes);
         launch_zero_f64(J.tally, w->nreg * w->slots, c->aux);
-        if (J.ledger) launch_zero_f64(J.ledger, 3 * w->nhist, c->aux);
+        if (J.ledger) launch_zero_f64(J.ledger, OMC_LEDGER * w->nhist, c->aux);
         HIPCHK(hipMemcpyAsync(J.cdf, w->cdf.data(), 2 * OMC_INVDIM * sizeof(double), hipMemcpyHostToDevice, c->aux));
         DevJob d = w->tmpl;
         d.phantom = w->phantom;
@@ -832,7 +832,8 @@ static int finish_deliver(omc_dev_ctx *c, int j)
         }
     }
     if (k && k->ledger && J.ledger &&
-        hipMemcpy(k->ledger + 3 * w->ledger_off, J.ledger, J.ledger_bytes, hipMemcpyDeviceToHost) != hipSuccess)
+        hipMemcpy(k->ledger + OMC_LEDGER * w->ledger_off, J.ledger, J.ledger_bytes, hipMemcpyDeviceToHost) !=
+            hipSuccess)
         return fail(OMC_EDEVICE, "copy ledger");
     if (k && k->sparse) {
         k->rows[w->index].assign(c->h_rows, c->h_rows + J.nnz);
@@ -1708,6 +1709,29 @@ int omc_dev_tune_beamlets(omc_dev_ctx *c, const omc_source_beamlets *src, const 
         k.beamlets = true;     // no sums wanted: only the transport is timed
         return run_sink(c, k, works, nullptr);
     }, hist, best, hps);
+}
+
+int omc_dev_math_eval(int device, int precision, int fn, int64_t n, const double *x, const double *y, double *out)
+{
+    if (!x || !out || n < 0 || fn < 0 || fn > 5) return fail(OMC_EARG, "bad argument");
+    if (precision != OMC_PREC_FP64 && precision != OMC_PREC_FP32) return fail(OMC_EARG, "precision %d", precision);
+    if (n == 0) return OMC_OK;
+    HIPCHK(hipSetDevice(device));
+    double *d = nullptr;
+    const size_t bytes = sizeof(double) * (size_t)n;
+    HIPCHK(hipMalloc((void **)&d, 3 * bytes));
+    int rc = OMC_OK;
+    if (hipMemcpy(d, x, bytes, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d + n, y ? y : x, bytes, hipMemcpyHostToDevice) != hipSuccess)
+        rc = fail(OMC_EDEVICE, "copy arguments");
+    if (!rc) {
+        if (precision == OMC_PREC_FP64) Launch<double>::math_eval(fn, n, d, d + n, d + 2 * n, nullptr);
+        else Launch<float>::math_eval(fn, n, d, d + n, d + 2 * n, nullptr);
+        if (hipGetLastError() != hipSuccess || hipMemcpy(out, d + 2 * n, bytes, hipMemcpyDeviceToHost) != hipSuccess)
+            rc = fail(OMC_EDEVICE, "math evaluation");
+    }
+    (void)hipFree(d);
+    return rc;
 }
 
 int omc_dev_submit(omc_dev_ctx *c, const omc_dev_request *req)

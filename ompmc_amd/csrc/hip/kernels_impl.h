@@ -140,4 +140,31 @@ void Launch<R>::electron(const KArgs<R> &a, int grid, hipStream_t s)
     hipLaunchKernelGGL(k_electron<R>, dim3(grid), dim3(RNG_LANES), OMC_RNG_DEPTH_E ? RNG_RING_BYTES : 0, s, a);
 }
 
+// The transport's math functions (m_log ... m_pow in the precision R, as
+// the kernels of this build compile them) on n arguments: a diagnostic of
+// where the device's arithmetic can differ from the host C library's.
+template <typename R>
+__global__ void k_math_eval(int fn, int64_t n, const double *x, const double *y, double *out)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const R a = (R)x[i], b = (R)y[i];
+    R r;
+    switch (fn) {
+    case 0: r = m_log(a); break;
+    case 1: r = m_exp(a); break;
+    case 2: r = m_sqrt(a); break;
+    case 3: r = m_sin(a); break;
+    case 4: r = m_cos(a); break;
+    default: r = m_pow(a, b); break;
+    }
+    out[i] = (double)r;
+}
+
+template <typename R>
+void Launch<R>::math_eval(int fn, int64_t n, const double *x, const double *y, double *out, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_math_eval<R>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, fn, n, x, y, out);
+}
+
 }  // namespace omc

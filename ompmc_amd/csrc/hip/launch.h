@@ -15,6 +15,7 @@ template <typename R> struct Launch {
                        uint32_t ntot, const Queue<R> &qp, const Queue<R> &qe, DevStatus *st, hipStream_t s);
     static void photon(const KArgs<R> &a, int grid, hipStream_t s);
     static void electron(const KArgs<R> &a, int grid, hipStream_t s);
+    static void math_eval(int fn, int64_t n, const double *x, const double *y, double *out, hipStream_t s);
 };
 
 // precision-independent helpers (defined in k64.hip)

@@ -57,7 +57,7 @@ template <typename R> struct LaneGeo {
         g = &lds[J.phantom];
         uint32_t batch = (uint32_t)((uint64_t)p.hist / (uint64_t)J.nperbatch);
         tally = glob(J.tally) + (size_t)batch * (size_t)g->nreg;
-        ledger = J.ledger ? glob(J.ledger) + 3 * (size_t)p.hist : nullptr;
+        ledger = J.ledger ? glob(J.ledger) + OMC_LEDGER * (size_t)p.hist : nullptr;
     }
     __device__ __forceinline__ int medium(int ir) const { int m = glob(g->med)[ir]; return m == 0xff ? -1 : m; }
     __device__ __forceinline__ R rho(int ir) const { return glob(g->rhof)[ir]; }
@@ -85,7 +85,11 @@ template <typename R> struct Scorer {
     {
         if (ir >= 0) {
             atomicAdd((double *)(G.tally + ir), acc);
-            if (G.ledger) atomicAdd((double *)(G.ledger + (ir > 0 ? 0 : 1)), acc);
+            if (G.ledger) {
+                atomicAdd((double *)(G.ledger + (ir > 0 ? 0 : 1)), acc);
+                // path checksum: energy times the region it went to
+                if (ir > 0) atomicAdd((double *)(G.ledger + 3), acc * (double)ir);
+            }
         }
         ir = -1;
         acc = 0.0;

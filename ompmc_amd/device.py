@@ -12,6 +12,10 @@ from . import _native as N
 FP64, FP32 = 0, 1
 _dp = C.POINTER(C.c_double)
 
+# entries per history of the ledger (OMC_LEDGER): deposited in the phantom,
+# deposited in region 0, carried out by photons, sum(energy * region)
+LEDGER = 4
+
 
 class DevOpts(C.Structure):
     _fields_ = [("device", C.c_int), ("precision", C.c_int), ("hbm_budget", C.c_int64), ("chunk", C.c_int64),
@@ -79,6 +83,7 @@ def lib():
         L.omc_dev_run_dosxyz.argtypes = [C.c_void_p, C.POINTER(N.SourceDosxyz), C.POINTER(RunParams), _dp, _dp]
         L.omc_dev_tune.argtypes = [C.c_void_p, C.POINTER(N.SourceDosxyz), C.POINTER(RunParams), C.c_int64,
                                    C.POINTER(DevOpts), C.POINTER(C.c_double)]
+        L.omc_dev_math_eval.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int64, _dp, _dp, _dp]
         L.omc_dev_tune_beamlets.argtypes = [C.c_void_p, C.POINTER(N.SourceBeamlets), C.POINTER(C.c_int64),
                                             C.POINTER(RunParams), C.POINTER(DevOpts), C.POINTER(C.c_double)]
         _i64p = C.POINTER(C.c_int64)
@@ -140,7 +145,7 @@ class Engine:
 
     def run_dosxyz(self, pid, src, nperbatch, nbatch, seed=(97, 33), hist0=0, nreg=None, ledger=False, nsplit=1):
         stats = DevStats()
-        led = np.zeros(3 * nperbatch * nbatch) if ledger else None
+        led = np.zeros(LEDGER * nperbatch * nbatch) if ledger else None
         rp = RunParams(pid, nsplit, (C.c_uint32 * 2)(*seed), hist0, nperbatch, nbatch,
                        led.ctypes.data_as(_dp) if ledger else None, C.pointer(stats))
         s = np.zeros(nreg)
@@ -172,13 +177,13 @@ class Engine:
 
     def run_beamlets(self, pid, src, nhist, nbatch, seed=(97, 33), hist0=0, nsplit=1, ledger=False):
         """Batch sums of every beamlet: arrays (nbeamlets, nreg); with
-        ledger=True also the per-history ledger (3 values per history,
+        ledger=True also the per-history ledger (LEDGER values per history,
         beamlet after beamlet) as a fourth value."""
         nreg = self.nreg[pid]
         nh = np.ascontiguousarray(nhist, dtype=np.int64)
         stats = DevStats()
         nrun = int(sum(max(1, int(n) // nbatch) * nbatch for n in nh))
-        led = np.zeros(3 * nrun) if ledger else None
+        led = np.zeros(LEDGER * nrun) if ledger else None
         rp = RunParams(pid, nsplit, (C.c_uint32 * 2)(*seed), hist0, 0, nbatch,
                        led.ctypes.data_as(_dp) if ledger else None, C.pointer(stats))
         s = np.zeros((src.nbeamlets, nreg))
@@ -231,3 +236,18 @@ class Engine:
         s2 = np.empty(nreg)
         check(lib().omc_dev_poll(self.ctx, ticket, C.byref(res), s.ctypes.data_as(_dp), s2.ctypes.data_as(_dp)))
         return (res, s, s2) if res.done and res.status == 0 else (res, None, None)
+
+
+MATH = {"log": 0, "exp": 1, "sqrt": 2, "sin": 3, "cos": 4, "pow": 5}
+
+
+def math_eval(fn, x, y=None, precision=FP64, device=0):
+    """omc_dev_math_eval: the transport's elementary function `fn` on the
+    device, for the arguments x (and y for pow)."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    yy = np.ascontiguousarray(y if y is not None else x, dtype=np.float64)
+    out = np.empty_like(x)
+    check(lib().omc_dev_math_eval(device, precision, MATH[fn], x.size, x.ctypes.data_as(_dp),
+                                  yy.ctypes.data_as(_dp), out.ctypes.data_as(_dp)))
+    return out
+

@@ -243,8 +243,10 @@ static void ausgab(ora_ctx *c, double edep)
     double e = p->wt * edep;
     c->cnt.n_dep++;
     if (c->endep) c->endep[p->ir] += e;
-    if (c->ledger && c->hid >= 0 && c->hid < c->ledger_n)
-        c->ledger[3 * c->hid + (p->ir > 0 ? 0 : 1)] += e;
+    if (c->ledger && c->hid >= 0 && c->hid < c->ledger_n) {
+        c->ledger[4 * c->hid + (p->ir > 0 ? 0 : 1)] += e;
+        if (p->ir > 0) c->ledger[4 * c->hid + 3] += e * (double)p->ir;
+    }
 }
 
 /* ========================================================= rotations */
@@ -662,7 +664,7 @@ start_mfp_loop:
             S[np].z += ustep * S[np].w;
             if (idisc > 0) {
                 if (c->ledger && c->hid >= 0 && c->hid < c->ledger_n)
-                    c->ledger[3 * c->hid + 2] += S[np].wt * S[np].e;
+                    c->ledger[4 * c->hid + 2] += S[np].wt * S[np].e;
                 np -= 1;
                 c->np = np;
                 if (np < 0) return;

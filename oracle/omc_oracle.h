@@ -74,8 +74,9 @@ int64_t  ora_tape_pos(const ora_ctx *c);
 /* Next uniform in [0,1) of the active mode (exposed for known-answer tests). */
 double   ora_random(ora_ctx *c);
 void     ora_get_counters(const ora_ctx *c, ora_counters *out);
-/* Optional per-history energy ledger (deposited in phantom, deposited
- * outside / escaped charged, photon energy leaving) indexed by history id. */
+/* Optional per-history ledger, 4 entries per history id: energy deposited
+ * in the phantom, deposited outside (region 0), carried out by photons, and
+ * the path checksum sum(energy * region) over deposits in the phantom. */
 void     ora_set_ledger(ora_ctx *c, double *ledger3, int64_t n);
 
 /* omc_dosxyz batch loop (omc_dosxyz.c:1206-1263): nbatch batches of

@@ -79,11 +79,15 @@ def test_c5_fp32_matches_fp64(oracle, D, tmp_path):
     DS.assert_equal_accuracy(r, "C5 FP32 vs FP64")
 
 
-@pytest.mark.parametrize("precision,tol", [(0, 1e-12), (1, 2e-6)])
+@pytest.mark.parametrize("precision,tol", [(0, 1e-12), (1, 1e-5)])
 def test_c5_energy_conserved_per_history(oracle, D, tmp_path, precision, tol):
     """Mono 6 MeV photons in the five C5 beams: every history's energy is
     deposited in the phantom, in region 0 or leaves it (the ledger), and
-    the deposits in the phantom are the tallies."""
+    the deposits in the phantom are the tallies.  FP32 carries a particle's
+    energy in FP32 and rounds it at every energy-loss step (half an ulp of
+    4-8 MeV is 2.4e-7 MeV); the electrons of a 6 MeV history take a few
+    hundred steps through the 2 mm voxels, so its total may drift by some
+    1e-6 and at most ~1e-4: the FP32 bound is 1e-5 relative."""
     prob, src = beam_problem(oracle, tmp_path, cfg.config_phantom("C5"), mono=6.0)
     try:
         nh = np.full(src.nbeamlets, 40000, np.int64)
@@ -94,9 +98,10 @@ def test_c5_energy_conserved_per_history(oracle, D, tmp_path, precision, tol):
     finally:
         oracle.free_problem(prob)
     assert st.killed == 0 and st.woodcock_over == 0
-    led = led.reshape(-1, 3)
+    led = led.reshape(-1, 4)
     assert led.shape[0] == int(nh.sum())
-    assert np.allclose(led.sum(1), 6.0, rtol=tol, atol=tol)
+    dev = np.abs(led[:, :3].sum(1) - 6.0)
+    assert dev.max() <= tol * 6.0, f"max deviation {dev.max():.3e} MeV, {np.count_nonzero(dev > tol * 6)} histories"
     assert abs(led[:, :2].sum() - s.sum()) < 1e-9 * s.sum()
 
 

@@ -67,7 +67,7 @@ def test_oracle_philox_energy_conservation(oracle, tmp_path):
         ctx = L.ora_create(C.byref(prob.tab), C.byref(prob.reg), C.byref(prob.ph), 1, oracle.PHILOX)
         try:
             L.ora_seed_philox(ctx, 1, 2)
-            led = np.zeros(3 * prob.nhist)
+            led = np.zeros(4 * prob.nhist)
             L.ora_set_ledger(ctx, led.ctypes.data_as(C.POINTER(C.c_double)), prob.nhist)
             s = np.zeros(prob.reg.nreg)
             s2 = np.zeros(prob.reg.nreg)
@@ -77,7 +77,7 @@ def test_oracle_philox_energy_conservation(oracle, tmp_path):
         finally:
             L.ora_destroy(ctx)
             oracle.free_problem(prob)
-        tot = led.reshape(-1, 3).sum(1)
+        tot = led.reshape(-1, 4)[:, :3].sum(1)
         assert np.allclose(tot, e_in, rtol=1e-12, atol=1e-12), case
 
 

@@ -72,7 +72,7 @@ def test_delta_tracking_integral_energy_per_history(oracle, D, tmp_path, case, p
     finally:
         oracle.free_problem(prob)
     assert stw.woodcock_over == 0 and stw.killed == 0
-    dv, dw = lv.reshape(-1, 3)[:, 0], lw.reshape(-1, 3)[:, 0]
+    dv, dw = lv.reshape(-1, 4)[:, 0], lw.reshape(-1, 4)[:, 0]
     se = np.sqrt(dv.var() / nh + dw.var() / nh)
     assert abs(dw.mean() - dv.mean()) < 4 * se, (dw.mean(), dv.mean(), se)
 
@@ -88,10 +88,10 @@ def test_delta_tracking_conserves_energy_per_history(oracle, D, tmp_path, precis
         s, _, st, led = _run(D, prob, precision, 1, (5, 9), ledger=True)
     finally:
         oracle.free_problem(prob)
-    led = led.reshape(-1, 3)
+    led = led.reshape(-1, 4)
     assert st.woodcock_over == 0 and st.killed == 0
     assert st.photon_steps < 10 * prob.nhist
-    assert np.allclose(led.sum(1), 2.0, rtol=tol, atol=tol)
+    assert np.allclose(led[:, :3].sum(1), 2.0, rtol=tol, atol=tol)
     assert abs(led[:, :2].sum() - s.sum()) < 1e-9 * s.sum()
 
 
@@ -133,7 +133,7 @@ def test_delta_tracking_beamlets_match_voxel_stepping(oracle, D, tmp_path):
     cfg.write_inp(tmp_path / "bl", cfg.dosxyz_inp_items(tmp_path, pf, ncase=100, spectrum_file=spec))
     prob = oracle.load_problem(tmp_path / "bl")
     try:
-        bs = cfg.beamlets(gantry_deg=(0.0, 90.0), field=6.0, bixel=3.0, sad=100.0)
+        bs = cfg.beamlets(gantry_deg=(0.0, 90.0), field=8.0, bixel=4.0, sad=100.0)
         src = N.beamlet_source(prob.src.spec, bs.ibeam, bs.src, bs.corner, bs.side1, bs.side2)
         nh, nb = [2_000_000] * src.nbeamlets, 100
         out = {}
@@ -187,4 +187,4 @@ def test_delta_tracking_through_vacuum_voxels(D, tmp_path):
     finally:
         P.free_problem(prob)
     assert stw.woodcock_over == 0 and stw.killed == 0
-    assert np.allclose(lw.reshape(-1, 3).sum(1), 3.0, rtol=2e-6, atol=2e-6)
+    assert np.allclose(lw.reshape(-1, 4)[:, :3].sum(1), 3.0, rtol=2e-6, atol=2e-6)
