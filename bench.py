@@ -500,6 +500,21 @@ def latency_section(prob, src, nreq=48, load=0.7, seed=77, nbatch=10, target=0.0
                 "sigma_at_dmax_max": float(max(sig)), "sigma_at_dmax_median": float(np.median(sig)),
                 "requests_extended": int(sum(p > 1 for p in parts))}
 
+    # a lone job: one 2e4-history beamlet request on an idle engine, repeated
+    lone = {}
+    for name, prec in (("fp32", D.FP32), ("fp64", D.FP64)):
+        with D.Engine(prec) as eng:
+            eng.upload_media(prob.tab)
+            pid = eng.upload_phantom(prob.ph, prob.reg)
+            wait_done(eng, eng.submit(pid, src, 20000, nbatch=nbatch, seed=(2, 2), kind=1, beamlet=0))
+            ms = []
+            for k in range(12):
+                res = wait_done(eng, eng.submit(pid, src, 20000, nbatch=nbatch, seed=(3, k), kind=1,
+                                                beamlet=int(beamlets[k % nreq])))
+                ms.append(res.finish_ms - res.submit_ms)
+        lone[name] = {"histories": 20000, "median_ms": float(np.median(ms)), "min_ms": float(min(ms)),
+                      "max_ms": float(max(ms))}
+
     # FP64 closed-loop capacity on the first 8 requests
     with D.Engine(D.FP64) as eng:
         eng.upload_media(prob.tab)
@@ -513,7 +528,7 @@ def latency_section(prob, src, nreq=48, load=0.7, seed=77, nbatch=10, target=0.0
         cap64 = 8 / (time.perf_counter() - t0)
     rate = load * cap64
     arrivals = np.cumsum(rng.exponential(1.0 / rate, size=nreq))
-    out = {"target_sigma_at_dmax": target, "pilot_histories_per_beamlet": npil,
+    out = {"lone_job": lone, "target_sigma_at_dmax": target, "pilot_histories_per_beamlet": npil,
            "fp64_capacity_requests_per_s": cap64, "load_of_fp64_capacity": load, "offered_rate_per_s": rate,
            "sizing": "pilot table per beamlet (1/sqrt(N), 25% margin); a request ending above the target is "
                      "extended by whole batches until it meets it; latency = first submit -> final result"}
