@@ -84,10 +84,6 @@ typedef struct {
                                 step.  0 = automatic (on for the FP32
                                 engine, off for FP64, which keeps its
                                 history-by-history parity), > 0 on, < 0 off */
-    int     msdist_div;      /* a charged-particle wavefront runs its waiting
-                                condensed-history steps (multiple
-                                scattering) once 1/msdist_div of its live
-                                lanes wait for one; 0 = automatic */
 } omc_dev_opts;
 
 /* Device counters of one run (all sums over the run). */
@@ -207,7 +203,7 @@ int  omc_dev_beamlets_sparse(omc_dev_ctx *ctx, const omc_source_beamlets *src,
  *   pipelines  2, 4, 6, 8 (at most GPU_MAX_HW_QUEUES - 1 when that is set)
  *   chunk      1M, 2M, 4M, 8M live particles per pipeline (within the budget)
  *   yield_trips 1024, 4096, 16384;  drain_lanes 16, 32, 48;  drain_trips 8, 16, 32
- *   event_div  1, 2, 4;  fetch_min 1, 4, 8, 16;  msdist_div 1, 2, 3, 4
+ *   event_div  1, 2, 4;  fetch_min 1, 4, 8, 16
  * Each setting is the better of two runs.  The histories, and so the
  * results, do not depend on any of these.  The context keeps the fastest
  * settings; *best receives them (device, precision and budget as they were)

@@ -97,9 +97,6 @@ constexpr int64_t drain_min_default(int cus, int precision)
 // the FP32 slab ran 133 / 140 / 144 / 145 M histories/s at fetch thresholds
 // of 16 / 4 / 8 / 1 lanes (FP64 and the beamlet batch within noise).
 constexpr uint32_t DEFAULT_EVENT_DIV = 2;
-// Condensed-history steps of the charged-particle kernel wait the same way
-// for half the live lanes of their wavefront.
-constexpr uint32_t DEFAULT_MSDIST_DIV = 2;
 constexpr uint32_t DEFAULT_FETCH_MIN = 1;
 
 struct PhantomSlot {
@@ -217,7 +214,7 @@ struct omc_dev_ctx {
     uint32_t yield_trips = 0;
     uint32_t drain_lanes = 0, drain_trips = 0;
     int64_t drain_min = 0;            // smallest launch input that drains
-    uint32_t event_div = 0, fetch_min = 0, msdist_div = 0;
+    uint32_t event_div = 0, fetch_min = 0;
     int cus = 256;
     std::chrono::steady_clock::time_point t0;
 
@@ -975,7 +972,6 @@ template <typename R> static int launch_generation(omc_dev_ctx *c, int pi)
     A.drain_trips = c->drain_trips;
     A.drain_min = (uint32_t)std::min<int64_t>(c->drain_min, UINT32_MAX);
     A.event_div = c->event_div;
-    A.msdist_div = c->msdist_div;
     A.fetch_min = c->fetch_min;
     A.woodcock = c->woodcock ? 1u : 0u;
     const int max_grid = c->cus * 8;
@@ -1297,7 +1293,6 @@ int omc_dev_create(omc_dev_ctx **out, const omc_dev_opts *opts)
     c->drain_trips = o.drain_trips > 0 ? (uint32_t)o.drain_trips : DEFAULT_DRAIN_TRIPS;
     c->event_div = o.event_div > 0 ? (uint32_t)o.event_div : DEFAULT_EVENT_DIV;
     c->fetch_min = o.fetch_min > 0 ? (uint32_t)std::min(o.fetch_min, 64) : DEFAULT_FETCH_MIN;
-    c->msdist_div = o.msdist_div > 0 ? (uint32_t)std::min(o.msdist_div, 64) : DEFAULT_MSDIST_DIV;
     hipDeviceProp_t prop;
     if (hipGetDeviceProperties(&prop, o.device) == hipSuccess) c->cus = prop.multiProcessorCount;
     c->drain_min = o.drain_min > 0 ? o.drain_min : o.drain_min < 0 ? 0 : drain_min_default(c->cus, c->precision);
@@ -1483,13 +1478,13 @@ namespace {
 struct TuneSetting {
     int pipes;
     int64_t chunk;
-    int yield_trips, drain_lanes, drain_trips, event_div, fetch_min, msdist_div;
+    int yield_trips, drain_lanes, drain_trips, event_div, fetch_min;
 };
 
 TuneSetting current_setting(const omc_dev_ctx *c)
 {
     return { (int)c->pipes.size(), c->chunk, (int)c->yield_trips, (int)c->drain_lanes, (int)c->drain_trips,
-             (int)c->event_div, (int)c->fetch_min, (int)c->msdist_div };
+             (int)c->event_div, (int)c->fetch_min };
 }
 
 int apply_setting(omc_dev_ctx *c, const TuneSetting &t)
@@ -1500,7 +1495,6 @@ int apply_setting(omc_dev_ctx *c, const TuneSetting &t)
     c->drain_trips = (uint32_t)t.drain_trips;
     c->event_div = (uint32_t)t.event_div;
     c->fetch_min = (uint32_t)t.fetch_min;
-    c->msdist_div = (uint32_t)t.msdist_div;
     return OMC_OK;
 }
 
@@ -1529,7 +1523,7 @@ static int tune_impl(omc_dev_ctx *c, const std::function<int()> &run, int64_t hi
         }
         return OMC_OK;
     };
-    static const std::vector<int64_t> axes[8] = {
+    static const std::vector<int64_t> axes[7] = {
         { 2, 4, 6, 8 },                              // pipelines
         { 1 << 20, 1 << 21, 1 << 22, 1 << 23 },      // live particles per pipeline
         { 1024, 4096, 16384 },                       // yield budget (trips)
@@ -1537,16 +1531,15 @@ static int tune_impl(omc_dev_ctx *c, const std::function<int()> &run, int64_t hi
         { 8, 16, 32 },                               // drain trips
         { 1, 2, 4 },                                 // photon event threshold divisor
         { 1, 4, 8, 16 },                             // fetch threshold (lanes)
-        { 1, 2, 3, 4 },                              // multiple-scattering threshold divisor
     };
     TuneSetting bst = current_setting(c);
     double bv = 0;
     int rc = measure(bst, bv);
-    for (int a = 0; a < 8 && !rc; ++a) {
+    for (int a = 0; a < 7 && !rc; ++a) {
         for (int64_t val : axes[a]) {
             TuneSetting t = bst;
-            int *fields[8] = { &t.pipes, nullptr, &t.yield_trips, &t.drain_lanes, &t.drain_trips, &t.event_div,
-                               &t.fetch_min, &t.msdist_div };
+            int *fields[7] = { &t.pipes, nullptr, &t.yield_trips, &t.drain_lanes, &t.drain_trips, &t.event_div,
+                               &t.fetch_min };
             if (a == 1) {
                 if (t.chunk == val) continue;
                 t.chunk = val;
@@ -1580,7 +1573,6 @@ static int tune_impl(omc_dev_ctx *c, const std::function<int()> &run, int64_t hi
     best->woodcock = c->woodcock ? 1 : -1;
     best->event_div = (int)c->event_div;
     best->fetch_min = (int)c->fetch_min;
-    best->msdist_div = (int)c->msdist_div;
     *hps = bv;
     return OMC_OK;
 }

@@ -37,8 +37,6 @@ template <typename R> struct KArgs {
     uint32_t drain_trips;    // ...if they have taken this many trips here...
     uint32_t drain_min;      // ...and the launch's input holds at least this many
     uint32_t event_div;      // photon events run once 1/event_div of the live lanes wait
-    uint32_t msdist_div;     // condensed-history steps (multiple scattering)
-                             // run once 1/msdist_div of the live lanes wait
     uint32_t fetch_min;      // free lanes take new particles once this many are free
     uint32_t woodcock;       // photon flights by delta tracking where the phantom allows
 };
@@ -538,14 +536,9 @@ __device__ __forceinline__ bool photon_flight(const KArgs<R> &A, Particle<R> &p,
 
 // ================================================== electrons/positrons ====
 template <typename R> struct ElectronLane {
-    int phase;               // 0 entry, 1 head of tstep loop, 2 ustep iteration,
-                             // 3 condensed-history step decided, multiple
-                             // scattering next (ustep holds tustep, de the
-                             // energy loss, zdemfp whether the step ends the
-                             // tstep's path); a lane in phase 3 never yields,
-                             // so none of this is saved
-    int imed, medold, lelke, compute_tstep, zdemfp;
-    R rhof, eke, elke, demfp, sig0, ustep, de;
+    int phase;               // 0 entry, 1 head of tstep loop, 2 ustep iteration
+    int imed, medold, lelke, compute_tstep;
+    R rhof, eke, elke, demfp, sig0, ustep;
 
     __device__ __forceinline__ void save(const Queue<R> &q, uint32_t i) const
     {
@@ -764,45 +757,16 @@ __device__ __forceinline__ int electron_iteration(const KArgs<R> &A, Particle<R>
     const int iq = p.iq;
     const int qel = (1 + iq) / 2;
     R eie = p.e;
-    int irl = p.ir, imed;
-    bool call_howfar, do_single = false, called_msdist = false, tstep_used = false;
-    double total_tstep = 0.0;             // fresh every iteration (ompmc.c:4787)
-    R tustep, range = 0, tvstep = 0, de = 0, tperp = 0;
-    const ElecCsda *csb = nullptr;
-    const MediumScalars *M = nullptr;
-    R ustep;
-    R xf = 0, yf = 0, zf = 0, uf = 0, vf = 0, wf = 0;
 
-    if (s.phase == 3) {
-        // A condensed-history step decided in an earlier trip of this lane
-        // (the wavefront runs the multiple-scattering branch for all lanes
-        // that wait for it at once): msdist, then the rest of the ustep
-        // iteration.  No random number has been drawn since the decision, so
-        // the particle's stream is consumed in the reference's order.
-        imed = s.imed;
-        M = &T.small->med[imed];
-        csb = T.csda + (imed * 2 + qel) * MXEKE;
-        tustep = s.ustep;
-        de = s.de;
-        tvstep = tustep;
-        tstep_used = s.zdemfp != 0;
-        call_howfar = false;
-        called_msdist = true;
-        Particle<R> q = p;
-        count_lane_event(EV_MSDIST);
-        ustep = msdist<R>(T, q, imed, s.rhof, de, tustep, s.eke);
-        p.rng = q.rng;
-        xf = q.x; yf = q.y; zf = q.z; uf = q.u; vf = q.v; wf = q.w;
-        s.phase = 2;
-    } else {
     if (s.phase == 0) {
+        int irl = p.ir;
         s.imed = G.medium(irl);
         s.rhof = G.rho(irl);
         s.medold = s.imed;
         if (eie <= (R)G.g->ecut[s.imed + 1]) { electron_cutoff<R>(A, p, G, sc, eie); return E_DONE; }
         s.phase = 1;
     }
-    M = s.imed >= 0 ? &T.small->med[s.imed] : nullptr;
+    const MediumScalars *M = s.imed >= 0 ? &T.small->med[s.imed] : nullptr;
     if (s.phase == 1) {
         s.compute_tstep = 1;
         s.eke = eie - rm;
@@ -827,8 +791,14 @@ __device__ __forceinline__ int electron_iteration(const KArgs<R> &A, Particle<R>
 
     // ---- one ustep iteration ----
     nsteps++;
-    imed = s.imed;
-    R tstep;
+    int irl = p.ir;
+    int imed = s.imed;
+    bool call_howfar, do_single = false, called_msdist = false;
+    R tstep, tustep, range = 0, tvstep = 0, de = 0, tperp = 0;
+    double total_tstep = 0.0;             // fresh every iteration (ompmc.c:4787)
+    const ElecCsda *csb = nullptr;
+    R ustep;
+    R xf = 0, yf = 0, zf = 0, uf = 0, vf = 0, wf = 0;
     if (imed == -1) {
         tstep = (R)10.0E8;
         ustep = tstep;
@@ -896,18 +866,19 @@ __device__ __forceinline__ int electron_iteration(const KArgs<R> &A, Particle<R>
         R skindepth = (R)3 * ssmfp;
         tustep = m_fmin(tustep, m_fmax(tperp, skindepth));
         if ((tustep <= tperp) && (tustep > skindepth)) {
-            // condensed-history step: the energy loss now, multiple
-            // scattering in a later trip together with the other lanes of
-            // the wavefront that wait for it (phase 3)
+            call_howfar = false;
+            do_single = false;
+            called_msdist = true;
             de = (R)eloss(csb, M->te, eke0, eke1, (double)rhof, (double)tustep, (double)range,
                           (double)s.eke, (double)s.elke, s.lelke);
-            s.ustep = tustep;
-            s.de = de;
-            // the end of the iteration tests total_tstep - tvstep * rhof
-            // (ompmc.c:5334-5339); tvstep = tustep on this path
-            s.zdemfp = (total_tstep - (double)tustep * (double)rhof < 1.0E-9) ? 1 : 0;
-            s.phase = 3;
-            return E_ALIVE;
+            tvstep = tustep;
+            // msdist moves a copy; the original position is kept until
+            // after ausgab, which scores in the pre-step region
+            Particle<R> q = p;
+            count_lane_event(EV_MSDIST);
+            ustep = msdist<R>(T, q, imed, rhof, de, tustep, s.eke);
+            p.rng = q.rng;
+            xf = q.x; yf = q.y; zf = q.z; uf = q.u; vf = q.v; wf = q.w;
         } else {
             called_msdist = false;
             R rnno = urand<R>(p.rng);
@@ -927,7 +898,6 @@ __device__ __forceinline__ int electron_iteration(const KArgs<R> &A, Particle<R>
             ustep = tustep;
             call_howfar = !(ustep < tperp);
         }
-    }
     }
 
     int irold = p.ir, irnew = p.ir;
@@ -1038,7 +1008,8 @@ __device__ __forceinline__ int electron_iteration(const KArgs<R> &A, Particle<R>
     if (eie <= (R)G.g->ecut[s.imed + 1]) { electron_cutoff<R>(A, p, G, sc, eie); return E_DONE; }
     if (s.imed != s.medold) { s.phase = 1; return E_ALIVE; }
     s.demfp -= de * s.sig0;
-    if (called_msdist ? tstep_used : total_tstep - (double)tvstep * (double)rhof < 1.0E-9) s.demfp = (R)0.0;
+    total_tstep -= (double)tvstep * (double)rhof;
+    if (total_tstep < 1.0E-9) s.demfp = (R)0.0;
     if (s.demfp >= (R)EPSEMFP) return zero_ustep<R>(A, p, s, G, sc, nsteps, ncross);
 
     return electron_interaction<R>(A, p, s);
@@ -1192,17 +1163,9 @@ __global__ void __launch_bounds__(256, Waves<R>::electron) k_electron(KArgs<R> A
         }
         const unsigned long long live = __ballot(alive);
         if (live == 0) break;
-        const uint32_t nlive = (uint32_t)__popcll(live);
-        const bool drain = in_end >= A.drain_min && __ballot(exhausted) != 0 && nlive <= A.drain_lanes;
-        // Lanes that have decided on a condensed-history step wait (phase 3)
-        // until 1/msdist_div of the live lanes do, or no other lane can go
-        // on; then a trip runs multiple scattering for all of them together
-        // and nothing else, so msdist, the costliest branch of the ustep
-        // iteration, is not executed for a few lanes while the rest of the
-        // wavefront idles in it.
-        const uint32_t nms = (uint32_t)__popcll(__ballot(alive && s.phase == 3));
-        const bool ms_trip = nms * A.msdist_div >= nlive || nms == nlive;
-        if (alive && (s.phase == 3) == ms_trip) {
+        const bool drain =
+            in_end >= A.drain_min && __ballot(exhausted) != 0 && (uint32_t)__popcll(live) <= A.drain_lanes;
+        if (alive) {
             p.rng.top_up(OMC_RNG_DEPTH_E);
             int r = electron_iteration<R>(A, p, s, G, sc, nsteps, ncross);
             ++trips;
@@ -1210,8 +1173,7 @@ __global__ void __launch_bounds__(256, Waves<R>::electron) k_electron(KArgs<R> A
             if (r != E_DONE && trips > MAX_ITERS) {     // watchdog: no particle runs forever
                 atomicAdd(&A.status->killed, 1u);
                 r = E_DONE;
-            } else if (r != E_DONE && s.phase != 3 &&
-                       (here >= A.yield_trips || (drain && here >= A.drain_trips))) {
+            } else if (r != E_DONE && (here >= A.yield_trips || (drain && here >= A.drain_trips))) {
                 sc.flush(G);
                 yield_push<R>(A.qe, p, s, trips, A.status);
                 nyield++;

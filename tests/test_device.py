@@ -82,12 +82,30 @@ def replay_report(r):
             "max_voxel_diff_of_dmax": float(np.abs(sd - so).max() / np.abs(so).max()), "killed": int(st.killed)}
 
 
+def howfar_tie_allowance(n):
+    """Charged-particle howfar calls the device and the oracle may count
+    differently without any other difference.  The device's FP64 log, exp,
+    pow, sin and cos differ from glibc's by one ulp on 0.3-24% of arguments
+    (test_math.py), so a step length or lateral distance can differ in its
+    last bit; where that flips the reference's `ustep < tperp` test
+    (ompmc.c:5057) on a step that howfar then finds inside the voxel (the
+    distance along the direction is at least the perpendicular tperp), one
+    side calls howfar and the other does not, and the history is the same.
+    Measured: 1-2 calls in 1e4-5e4 per case; allowed: 1e-4 of the calls + 2."""
+    return int(1e-4 * n) + 2
+
+
 def assert_same_histories(r, what):
-    """Exact replay: every history's ledger row, every physics event count
-    and every voxel (to summation order) equal the oracle's."""
+    """Exact replay: every history's ledger row (energies and the path
+    checksum), every physics event count, and every voxel (to summation
+    order) equal the oracle's; the count of charged-particle howfar calls
+    within the allowance howfar_tie_allowance names the cause of."""
     rep = replay_report(r)
-    assert (rep["histories_differ"], rep["counters_differ"], rep["voxels_differ"], rep["killed"]) == (0, {}, 0, 0), \
-        f"{what}: {rep}"
+    diff = dict(rep["counters_differ"])
+    hf = diff.pop("howfar_charged", None)
+    assert (rep["histories_differ"], diff, rep["voxels_differ"], rep["killed"]) == (0, {}, 0, 0), f"{what}: {rep}"
+    if hf is not None:
+        assert abs(hf[0] - hf[1]) <= howfar_tie_allowance(hf[1]), f"{what}: {rep}"
 
 
 REPLAY = ["water2MeV", "slab6MV", "patient6MV", "electron6MeV", "positron6MeV", "pencil", "oversize", "C1",
