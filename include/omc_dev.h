@@ -168,6 +168,22 @@ typedef struct {
 int  omc_dev_run_dosxyz(omc_dev_ctx *ctx, const omc_source_dosxyz *src,
                         const omc_dev_run_params *rp, double *sum, double *sum2);
 
+/* The same batch loop for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc): no counterpart in the reference.  The job's batch tallies and
+ * spectrum live in a static device array of OMC_NOALLOC_SCRATCH_BYTES per
+ * GPU instead of memory the call allocates; the context, its particle
+ * queues, tables and phantoms are set up beforehand as usual, and this call
+ * makes no hipMalloc, hipFree or other allocating device call.  The tallies
+ * must fit: 8 * nreg * nbatch bytes plus 16 KiB, else OMC_ERANGE (the C2
+ * slab at 10 batches takes 21 MB).  No ledger (rp->ledger must be NULL), and
+ * no other work may be queued in the context.  Every call on one GPU uses
+ * that GPU's one array, so calls that overlap on a GPU (several contexts on
+ * it, from several threads) race; each call clears the tallies it uses. */
+#define OMC_NOALLOC_SCRATCH_BYTES (64u << 20)
+int  omc_dev_run_dosxyz_noalloc(omc_dev_ctx *ctx, const omc_source_dosxyz *src,
+                                const omc_dev_run_params *rp, double *sum, double *sum2);
+
 /* matRad dose influence: every beamlet of `src` in one device pass instead
  * of the serial beamlet loop (omc_matrad.c:1389-1414).  Beamlet b runs
  * nperbatch_b * rp->nbatch histories with nperbatch_b = nhist[b] / nbatch

@@ -26,6 +26,7 @@
 #include <deque>
 #include <functional>
 #include <map>
+#include <mutex>
 #include <thread>
 #include <vector>
 
@@ -34,6 +35,15 @@
 #include "transport.h"
 
 using namespace omc;
+
+// Device memory of omc_dev_run_dosxyz_noalloc: the batch tallies and the
+// spectrum of its one job live here instead of in memory the call
+// allocates.  Every GPU the library runs on holds its own copy for the life
+// of the process.  In host code `scratch` names a host-side stand-in, not
+// the device address: the address comes from hipGetSymbolAddress, once per
+// device (scratch_on), and only that address is handed to kernels and
+// copies.
+static __device__ __align__(256) unsigned char scratch[OMC_NOALLOC_SCRATCH_BYTES];
 
 namespace {
 
@@ -163,6 +173,7 @@ struct Work {
     int64_t nreg = 0;
     bool ledger = false;
     int64_t ledger_off = 0;       // first history of this work in the run's ledger
+    bool scratch = false;         // tally and spectrum go to the device's scratch array
     int state = W_WAITING;
     int status = OMC_OK;
     int job = -1;
@@ -188,6 +199,7 @@ enum { FIN_NONE = 0, FIN_QUEUED, FIN_FOLD, FIN_COLUMN };
 
 struct JobRec {
     bool used = false;
+    bool borrowed = false;        // tally and spectrum in the device's scratch array
     Work *w = nullptr;
     int64_t injected = 0;
     uint32_t pipes = 0;           // pipelines that may hold its particles
@@ -209,6 +221,7 @@ struct omc_dev_ctx {
     hipStream_t aux = nullptr;    // admission and finishing work
     int64_t budget = 0;
     int64_t used = 0;
+    void *scratch = nullptr;          // this device's scratch array (device address)
     int64_t chunk = 0;
     uint32_t qcap = 0;
     uint32_t yield_trips = 0;
@@ -652,8 +665,10 @@ template <typename R> static int upload_phantom(omc_dev_ctx *c, const omc_phanto
 static void release_job(omc_dev_ctx *c, int j)
 {
     JobRec &J = c->jobs[j];
-    pool_free(c, J.tally, J.tally_bytes);
-    if (J.cdf) dfree(c, J.cdf, 2 * OMC_INVDIM * sizeof(double));
+    if (!J.borrowed) {
+        pool_free(c, J.tally, J.tally_bytes);
+        if (J.cdf) dfree(c, J.cdf, 2 * OMC_INVDIM * sizeof(double));
+    }
     if (J.ledger) dfree(c, J.ledger, J.ledger_bytes);
     J = JobRec();
     while (c->jobs_hi > 0 && !c->jobs[c->jobs_hi - 1].used) c->jobs_hi--;
@@ -675,6 +690,7 @@ static void end_work(omc_dev_ctx *c, Work *w, int status)
 // Bytes a work needs on admission.
 static size_t work_bytes(const Work *w)
 {
+    if (w->scratch) return 0;
     return sizeof(double) * (size_t)w->nreg * w->slots + 2 * OMC_INVDIM * sizeof(double) +
            (w->ledger ? sizeof(double) * OMC_LEDGER * (size_t)w->nhist : 0);
 }
@@ -707,8 +723,16 @@ static int admit(omc_dev_ctx *c)
         JobRec &J = c->jobs[slot];
         J = JobRec();
         J.tally_bytes = sizeof(double) * (size_t)w->nreg * w->slots;
-        int rc = pool_alloc(c, (void **)&J.tally, J.tally_bytes);
-        if (!rc) rc = dalloc(c, (void **)&J.cdf, 2 * OMC_INVDIM * sizeof(double));
+        int rc = OMC_OK;
+        if (w->scratch) {
+            // the caller checked that both fit (omc_dev_run_dosxyz_noalloc)
+            J.borrowed = true;
+            J.tally = (double *)c->scratch;
+            J.cdf = (double *)((char *)c->scratch + ((J.tally_bytes + 255) & ~(size_t)255));
+        } else {
+            rc = pool_alloc(c, (void **)&J.tally, J.tally_bytes);
+            if (!rc) rc = dalloc(c, (void **)&J.cdf, 2 * OMC_INVDIM * sizeof(double));
+        }
         if (!rc && w->ledger) {
             J.ledger_bytes = sizeof(double) * OMC_LEDGER * (size_t)w->nhist;
             rc = dalloc(c, (void **)&J.ledger, J.ledger_bytes);
@@ -1227,6 +1251,25 @@ static int run_sink(omc_dev_ctx *c, RunSink &k, std::vector<Work> &works, omc_de
     return k.rc;
 }
 
+// Device address of the scratch array on `device` (current), looked up once
+// per device and kept.
+static int scratch_on(int device, void **out)
+{
+    static std::mutex mu;
+    static std::map<int, void *> addr;
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = addr.find(device);
+    if (it == addr.end()) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(OMC_EDEVICE, "hipGetSymbolAddress(scratch) on device %d: %s", device, hipGetErrorString(e));
+        it = addr.emplace(device, p).first;
+    }
+    *out = it->second;
+    return OMC_OK;
+}
+
 // ------------------------------------------------------------------ C ABI --
 extern "C" {
 
@@ -1284,7 +1327,10 @@ int omc_dev_create(omc_dev_ctx **out, const omc_dev_opts *opts)
     HIPCHK(hipGetDeviceCount(&ndev));
     if (o.device < 0 || o.device >= ndev) return fail(OMC_EDEVICE, "device %d not present (%d devices)", o.device, ndev);
     HIPCHK(hipSetDevice(o.device));
+    void *scratch_dev = nullptr;
+    OMC_TRY_DEV(scratch_on(o.device, &scratch_dev));
     omc_dev_ctx *c = new omc_dev_ctx();
+    c->scratch = scratch_dev;
     c->t0 = std::chrono::steady_clock::now();
     c->device = o.device;
     c->precision = o.precision;
@@ -1470,6 +1516,39 @@ int omc_dev_run_dosxyz(omc_dev_ctx *c, const omc_source_dosxyz *src, const omc_d
     k.sum = sum;
     k.sum2 = sum2;
     k.ledger = rp->ledger;
+    return run_sink(c, k, works, rp->stats);
+}
+
+int omc_dev_run_dosxyz_noalloc(omc_dev_ctx *c, const omc_source_dosxyz *src, const omc_dev_run_params *rp,
+                               double *sum, double *sum2)
+{
+    OMC_TRY_DEV(check_run(c, rp));
+    if (!src || !sum || !sum2) return fail(OMC_EARG, "null argument");
+    if (src->charge < -1 || src->charge > 1) return fail(OMC_EARG, "charge %d", src->charge);
+    if (rp->nperbatch < 1) return fail(OMC_EARG, "nperbatch must be >= 1");
+    if (rp->ledger) return fail(OMC_EARG, "the allocation-free run keeps no per-history ledger");
+    if (!c->scratch) return fail(OMC_EDEVICE, "no scratch array on device %d", c->device);
+    const int64_t nreg = c->ph[rp->phantom_id].nreg;
+    const size_t tally = sizeof(double) * (size_t)nreg * (size_t)rp->nbatch;
+    const size_t need = ((tally + 255) & ~(size_t)255) + 2 * OMC_INVDIM * sizeof(double);
+    if (need > OMC_NOALLOC_SCRATCH_BYTES)
+        return fail(OMC_ERANGE, "%d batches of %lld regions need %zu bytes of scratch, at most %zu", rp->nbatch,
+                    (long long)nreg, need, (size_t)OMC_NOALLOC_SCRATCH_BYTES);
+    if (any_running(c) || !c->waiting.empty())
+        return fail(OMC_EARG, "the allocation-free run needs a context with no queued work");
+    HIPCHK(hipSetDevice(c->device));
+    std::vector<Work> works(1);
+    Work &w = works[0];
+    dosxyz_work(w, src);
+    stream_fields(w, rp->seed, rp->hist0);
+    w.tmpl.nsplit = rp->nsplit;
+    OMC_TRY_DEV(set_histories(w, rp->nperbatch, rp->nbatch));
+    w.phantom = rp->phantom_id;
+    w.nreg = nreg;
+    w.scratch = true;
+    RunSink k;
+    k.sum = sum;
+    k.sum2 = sum2;
     return run_sink(c, k, works, rp->stats);
 }
 
