@@ -124,6 +124,11 @@ typedef struct {
     int64_t msdist;             /* condensed-history steps (msdist calls) */
     int64_t sscat;              /* single elastic scatterings (sscat calls) */
     int64_t howfar_charged;     /* howfar calls for charged particles */
+    /* Loop trips of the longest photon and charged-particle track of the
+     * run (one trip: a voxel step or flight, an interaction, one
+     * condensed-history step).  A launch lasts as long as its longest
+     * track, so these bound a small job's latency. */
+    int64_t longest_photon_trips, longest_electron_trips;
 } omc_dev_stats;
 
 /* Context: device, pipelines (each a stream with its own particle queues),

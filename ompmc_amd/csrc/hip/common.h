@@ -45,6 +45,26 @@ __device__ __forceinline__ void philox10(uint32_t c0, uint32_t c1, uint32_t c2, 
     out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
 }
 
+// The same function with its rounds kept as a loop: for the rare paths
+// inlined at every draw site (a block the ring does not hold, a secondary's
+// key).  Unrolled there, Philox was half of the charged-particle kernel's
+// instructions (about a hundred inlined copies), far more code than the
+// instruction cache a CU pair shares holds.
+__device__ __forceinline__ void philox10_rolled(uint32_t c0, uint32_t c1, uint32_t c2, uint32_t c3,
+                                                uint32_t k0, uint32_t k1, uint32_t out[4])
+{
+#pragma unroll 1
+    for (int r = 0; r < 10; ++r) {
+        uint32_t lo0 = 0xD2511F53u * c0, hi0 = __umulhi(0xD2511F53u, c0);
+        uint32_t lo1 = 0xCD9E8D57u * c2, hi1 = __umulhi(0xCD9E8D57u, c2);
+        uint32_t n0 = hi1 ^ c1 ^ k0;
+        uint32_t n2 = hi0 ^ c3 ^ k1;
+        c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+        k0 += 0x9E3779B9u; k1 += 0xBB67AE85u;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
 // Look-ahead of a lane's stream in LDS: RNG_RING Philox blocks per lane,
 // block b in slot b % RNG_RING of the lane's column.  A transport kernel
 // that uses it tops the ring up where the wavefront's live lanes are
@@ -93,7 +113,7 @@ struct Rng {
                 w0 = v.x; w1 = v.y; w2 = v.z; w3 = v.w;
             } else {
                 uint32_t o[4];
-                philox10(b, 0u, 0u, 0u, k0, k1, o);
+                philox10_rolled(b, 0u, 0u, 0u, k0, k1, o);
                 w0 = o[0]; w1 = o[1]; w2 = o[2]; w3 = o[3];
                 tail = b + 1;
             }
@@ -115,7 +135,7 @@ struct Rng {
     __device__ __forceinline__ void child(uint32_t &c0, uint32_t &c1, uint32_t tag = 1u) const
     {
         uint32_t o[4];
-        philox10(ctr, tag, 0u, 0u, k0, k1, o);
+        philox10_rolled(ctr, tag, 0u, 0u, k0, k1, o);
         c0 = o[0]; c1 = o[1];
     }
 };
@@ -153,6 +173,15 @@ __device__ __forceinline__ uint32_t wave_reserve(uint32_t *counter, uint32_t n_p
     if (lane_id() == leader) base = atomicAdd(counter, total);
     base = __shfl(base, leader);
     return base + lanes_below(m) * n_per_lane;
+}
+
+// Largest v over the lanes of a converged wavefront, into *p with one
+// atomicMax per wavefront.
+__device__ __forceinline__ void wave_max_to(uint32_t *p, uint32_t v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, o));
+    if (lane_id() == 0 && v) atomicMax(p, v);
 }
 
 // ------------------------------------------------------------- layouts ----

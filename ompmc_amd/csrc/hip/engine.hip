@@ -98,7 +98,7 @@ constexpr uint32_t DEFAULT_DRAIN_TRIPS = 16;
 // handed on every 16 trips.
 constexpr int64_t drain_min_default(int cus, int precision)
 {
-    return (int64_t)cus * 4 * (precision == OMC_PREC_FP64 ? 2 : 3) * 64;
+    return (int64_t)cus * 4 * (precision == OMC_PREC_FP64 ? 2 : 4) * 64;
 }
 // A photon wavefront runs its waiting events (interaction, re-entry) once
 // half its live lanes wait for one (slab sweep of 2/4/8/16: 2 first, 22%
@@ -1006,6 +1006,7 @@ template <typename R> static int launch_generation(omc_dev_ctx *c, int pi)
         A.qe = qec;
         A.in_count = P.d_counts + P.cur;
         A.fetch = P.d_counts + 4;
+        A.longest = P.d_counts + 7;
         HIPCHK(hipEventRecord(P.ev_ph0, s));
         Launch<R>::photon(A, (int)std::min<int64_t>(max_grid, ((int64_t)inP + 255) / 256), s);
         HIPCHK(hipEventRecord(P.ev_ph1, s));
@@ -1018,6 +1019,7 @@ template <typename R> static int launch_generation(omc_dev_ctx *c, int pi)
         A.qe = queue_view<R>(c, P, 2 + nxt);
         A.in_count = P.d_counts + 2 + P.cur;
         A.fetch = P.d_counts + 5;
+        A.longest = P.d_counts + 6;
         HIPCHK(hipEventRecord(P.ev_el0, s));
         Launch<R>::electron(A, (int)std::min<int64_t>(max_grid, ((int64_t)inE + 2 * (int64_t)inP + 255) / 256), s);
         HIPCHK(hipEventRecord(P.ev_el1, s));
@@ -1069,6 +1071,8 @@ static int complete_generation(omc_dev_ctx *c, int pi)
         S.electron_launches++;
     }
     S.generations++;
+    S.longest_electron_trips = std::max<int64_t>(S.longest_electron_trips, P.h->counts[6]);
+    S.longest_photon_trips = std::max<int64_t>(S.longest_photon_trips, P.h->counts[7]);
     S.photons_pushed += P.h->counts[P.cur];
     // electrons into the next queue, and those the photon kernel appended
     // to the queue this generation consumed

@@ -43,6 +43,8 @@ __global__ void k_gen_reset(uint32_t *counts, int nxt)
         counts[2 + nxt] = 0u;
         counts[4] = 0u;
         counts[5] = 0u;
+        counts[6] = 0u;         // longest charged-particle track of the generation
+        counts[7] = 0u;         // longest photon track
     }
 }
 

@@ -437,7 +437,7 @@ __device__ void brems(const DevTables<R> &T, Particle<R> &a, Particle<R> &b, int
         R aux3 = z2maxi / (y2tst + ((R)1.0 - y2tst) * z2maxi);
         rtest = rtest * aux3 * rejmax;
         y2tst = sq(aux3) - (R)1.0;
-        R aux3_4 = m_pow(aux3, (R)4.0);
+        R aux3_4 = m_pow4(aux3);
         R y2tst1 = esedei * y2tst / aux3_4;
         R aux4 = (R)16.0 * y2tst1 - rjarg2;
         R aux5 = rjarg1 - (R)4.0 * y2tst1;
@@ -514,7 +514,7 @@ __device__ void bhabha(const DevTables<R> &T, Particle<R> &a, Particle<R> &b, in
     R beta2 = (sq(e0) - (R)1.0) / sq(e0);
     R ep0 = (R)M.te / ekin, ep0c = (R)1.0 - ep0;
     R yp = (R)1.0 - (R)2.0 * yy;
-    R b4 = m_pow(yp, (R)3.0);
+    R b4 = m_pow3(yp);
     R b3 = b4 + sq(yp);
     R b2 = yp * ((R)3.0 + sq(yy));
     R b1 = (R)2.0 - sq(yy);

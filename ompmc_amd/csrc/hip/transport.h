@@ -39,6 +39,7 @@ template <typename R> struct KArgs {
     uint32_t event_div;      // photon events run once 1/event_div of the live lanes wait
     uint32_t fetch_min;      // free lanes take new particles once this many are free
     uint32_t woodcock;       // photon flights by delta tracking where the phantom allows
+    uint32_t *longest;       // trips of the longest track that ended in this launch (atomicMax)
 };
 
 // Per-lane geometry / scoring context for the particle the lane holds.  The
@@ -1018,15 +1019,16 @@ __device__ __forceinline__ int electron_iteration(const KArgs<R> &A, Particle<R>
 // ============================================================ kernels =====
 // Minimum waves per SIMD the register allocation of each transport kernel
 // must allow (the second __launch_bounds__ argument); values above what the
-// kernel needs make the compiler keep fewer values in VGPRs.  Chosen by a
-// sweep on the slab phantom (16M histories): FP32 photon 4 / electron 3
-// (no spills), FP64 photon 2 / electron 2 (a few spills, still faster than
-// one wave).
+// kernel needs make the compiler keep fewer values in VGPRs.  Chosen by
+// sweeps on the slab phantom: FP32 photon 4 (no spills) / electron 4 (128
+// VGPRs and a few spills; 3-4% more histories/s than 3 waves without
+// spills), FP64 photon 2 / electron 2 (a few spills, still faster than one
+// wave).
 #ifndef OMC_WAVES_P32
 #define OMC_WAVES_P32 4
 #endif
 #ifndef OMC_WAVES_E32
-#define OMC_WAVES_E32 3
+#define OMC_WAVES_E32 4
 #endif
 #ifndef OMC_WAVES_P64
 #define OMC_WAVES_P64 2
@@ -1060,7 +1062,7 @@ __global__ void __launch_bounds__(256, Waves<R>::photon) k_photon(KArgs<R> A)
     LaneGeo<R> G;
     Scorer<R> sc;
     bool alive = false, exhausted = false;
-    uint32_t nsteps = 0, ncross = 0, trips = 0, here = 0, nyield = 0;
+    uint32_t nsteps = 0, ncross = 0, trips = 0, here = 0, nyield = 0, longest = 0;
     const uint32_t in_end = min(*A.in_count, A.qin.cap);
     for (;;) {
         // free lanes take new particles together (one counter atomic and one
@@ -1115,10 +1117,15 @@ __global__ void __launch_bounds__(256, Waves<R>::photon) k_photon(KArgs<R> A)
                 yield_push<R>(A.qp, p, s, trips, A.status);
                 nyield++;
                 alive = false;
+                trips = 0;
             }
-            if (!alive) sc.flush(G);
+            if (!alive) {
+                sc.flush(G);
+                longest = max(longest, trips);
+            }
         }
     }
+    wave_max_to(A.longest, longest);
     sc.flush(G);
     if (nyield) atomicAdd(&A.status->yields, (unsigned long long)nyield);
     atomicAdd(&A.status->photon_steps, (unsigned long long)nsteps);
@@ -1137,7 +1144,7 @@ __global__ void __launch_bounds__(256, Waves<R>::electron) k_electron(KArgs<R> A
     LaneGeo<R> G;
     Scorer<R> sc;
     bool alive = false, exhausted = false;
-    uint32_t nsteps = 0, ncross = 0, trips = 0, here = 0, nyield = 0;
+    uint32_t nsteps = 0, ncross = 0, trips = 0, here = 0, nyield = 0, longest = 0;
     const uint32_t in_end = min(*A.in_count, A.qin.cap);
     for (;;) {
         // free lanes take new particles together (one counter atomic and one
@@ -1178,13 +1185,16 @@ __global__ void __launch_bounds__(256, Waves<R>::electron) k_electron(KArgs<R> A
                 yield_push<R>(A.qe, p, s, trips, A.status);
                 nyield++;
                 r = E_DONE;
+                trips = 0;
             }
             if (r == E_DONE) {
                 alive = false;
                 sc.flush(G);
+                longest = max(longest, trips);
             }
         }
     }
+    wave_max_to(A.longest, longest);
     sc.flush(G);
     if (nyield) atomicAdd(&A.status->yields, (unsigned long long)nyield);
     atomicAdd(&A.status->electron_steps, (unsigned long long)nsteps);

@@ -39,7 +39,7 @@ class DevStats(C.Structure):
                 ("pipe_idle_ms", C.c_double), ("woodcock_over", C.c_int64)] + [
                 (n, C.c_int64) for n in ("compton", "photo", "pair", "rayleigh", "brems", "moller", "bhabha",
                                          "annih_flight", "annih_rest", "fict_rejections", "msdist", "sscat",
-                                         "howfar_charged")]
+                                         "howfar_charged", "longest_photon_trips", "longest_electron_trips")]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -270,6 +270,26 @@ def test_schedule_and_reruns_do_not_change_histories(oracle, D, tmp_path, case, 
         assert np.allclose(led, out[0][1], rtol=1e-12, atol=1e-12)
 
 
+@pytest.mark.parametrize("case", ["electron6MeV", "slab6MV"])
+def test_longest_track_statistic(oracle, D, tmp_path, case):
+    """longest_electron_trips: the loop trips of the run's longest charged
+    track lie between the mean per track and the run's total.  A trip takes
+    one or two condensed-history steps (a zero-length step runs fused with
+    the next), and a track's last trip may take none (cut off on arrival),
+    so steps / (2 tracks) <= longest <= steps + tracks.  A charged track
+    starts with each charged primary, Moller or Bhabha secondary, Compton
+    or photo electron and pair member; the counts come from the replayed
+    run, whose events equal the oracle's."""
+    r = run_both(oracle, D, write_case(case, tmp_path, ncase=500))
+    assert_same_histories(r, case)
+    st = r["st"]
+    primaries_charged = st.histories if case.startswith("electron") else 0
+    tracks = primaries_charged + st.moller + st.bhabha + st.compton + st.photo + 2 * st.pair
+    assert 2 * st.longest_electron_trips * tracks >= st.electron_steps > 0
+    assert st.longest_electron_trips <= st.electron_steps + tracks
+    assert (st.longest_photon_trips > 0) == (st.photon_steps > 0)
+
+
 @pytest.mark.parametrize("energy,charge", [(3.0, 0), (6.0, -1), (4.0, 1)])
 def test_vacuum_voxels_inside_phantom(oracle, D, tmp_path, energy, charge):
     """Medium 0 voxels (vacuum) inside the grid: straight-line transport,
