@@ -84,6 +84,19 @@ typedef struct {
                                 step.  0 = automatic (on for the FP32
                                 engine, off for FP64, which keeps its
                                 history-by-history parity), > 0 on, < 0 off */
+    int64_t tally_bytes;     /* largest batch tally ([nbatch][nreg] FP64) a
+                                job holds whole.  A larger one is admitted
+                                batch by batch: each batch gets one grid,
+                                folded into the job's sum and sum of squares
+                                grids as soon as its last particle is gone
+                                (the batch method of accumEndep, batches
+                                summed in the order they end).  Batches in
+                                flight: as many as the limit leaves room
+                                for, but at least two and enough histories
+                                for half of pipelines x chunk; no split
+                                where that saves no grid.  Results equal the whole-tally ones to
+                                FP64 summation order.  0 = 512 MiB, < 0 =
+                                never split */
 } omc_dev_opts;
 
 /* Device counters of one run (all sums over the run). */

@@ -107,6 +107,10 @@ constexpr int64_t drain_min_default(int cus, int precision)
 // the FP32 slab ran 133 / 140 / 144 / 145 M histories/s at fetch thresholds
 // of 16 / 4 / 8 / 1 lanes (FP64 and the beamlet batch within noise).
 constexpr uint32_t DEFAULT_EVENT_DIV = 2;
+// Whole batch tallies up to 512 MiB per job (the C3 beamlet's 10 grids of the
+// 128x128x96 phantom take 126 MB); a C5-sized job (84 MB per grid) is
+// admitted batch by batch, four batches in flight beside its sum grids.
+constexpr int64_t DEFAULT_TALLY_CAP = (int64_t)512 << 20;
 constexpr uint32_t DEFAULT_FETCH_MIN = 1;
 
 struct PhantomSlot {
@@ -174,6 +178,12 @@ struct Work {
     bool ledger = false;
     int64_t ledger_off = 0;       // first history of this work in the run's ledger
     bool scratch = false;         // tally and spectrum go to the device's scratch array
+    // Admitted batch by batch (the whole tally would exceed the context's
+    // tally_bytes): at most split_k batches in flight, each its own job with
+    // one grid, folded into acc = [sum | sum of squares] when it ends.
+    int split_k = 0;
+    int next_batch = 0, batches_live = 0, batches_done = 0;
+    double *acc = nullptr;
     int state = W_WAITING;
     int status = OMC_OK;
     int job = -1;
@@ -201,6 +211,8 @@ struct JobRec {
     bool used = false;
     bool borrowed = false;        // tally and spectrum in the device's scratch array
     Work *w = nullptr;
+    int batch = -1;               // batch of a work admitted batch by batch (-1: the whole work)
+    int64_t nhist = 0;            // histories of this job
     int64_t injected = 0;
     uint32_t pipes = 0;           // pipelines that may hold its particles
     int64_t quantum = 0;          // histories per turn in the round robin
@@ -222,6 +234,7 @@ struct omc_dev_ctx {
     int64_t budget = 0;
     int64_t used = 0;
     void *scratch = nullptr;          // this device's scratch array (device address)
+    int64_t tally_cap = 0;            // largest whole batch tally of a job (bytes; 0 = no limit)
     int64_t chunk = 0;
     uint32_t qcap = 0;
     uint32_t yield_trips = 0;
@@ -676,6 +689,12 @@ static void release_job(omc_dev_ctx *c, int j)
 
 static void end_work(omc_dev_ctx *c, Work *w, int status)
 {
+    // a work admitted batch by batch has several jobs but ends once
+    if (w->state == W_DONE || w->state == W_FAILED) return;
+    if (w->acc) {
+        dfree(c, w->acc, 2 * sizeof(double) * (size_t)w->nreg);
+        w->acc = nullptr;
+    }
     w->state = status == OMC_OK ? W_DONE : W_FAILED;
     w->status = status;
     w->finish_ms = now_ms(c);
@@ -691,6 +710,9 @@ static void end_work(omc_dev_ctx *c, Work *w, int status)
 static size_t work_bytes(const Work *w)
 {
     if (w->scratch) return 0;
+    if (w->split_k)   // the next batch's grid, spectrum and ledger, and the sums with the first batch
+        return sizeof(double) * (size_t)w->nreg * (w->acc ? 1 : 3) + 2 * OMC_INVDIM * sizeof(double) +
+               (w->ledger ? sizeof(double) * OMC_LEDGER * (size_t)w->nperbatch : 0);
     return sizeof(double) * (size_t)w->nreg * w->slots + 2 * OMC_INVDIM * sizeof(double) +
            (w->ledger ? sizeof(double) * OMC_LEDGER * (size_t)w->nhist : 0);
 }
@@ -702,11 +724,15 @@ static int any_running(const omc_dev_ctx *c)
 }
 
 // Admit waiting works in order while their memory fits.  A work that cannot
-// fit even with nothing else admitted fails with OMC_EBUDGET.
+// fit even with nothing else admitted fails with OMC_EBUDGET.  A work
+// admitted batch by batch stays in line until its last batch is admitted;
+// while its batches in flight are at its limit, the works behind it go on.
 static int admit(omc_dev_ctx *c)
 {
-    while (!c->waiting.empty()) {
-        Work *w = c->waiting.front();
+    size_t at = 0;
+    while (at < c->waiting.size()) {
+        Work *w = c->waiting[at];
+        if (w->split_k && w->batches_live >= w->split_k) { ++at; continue; }
         int slot = -1;
         for (int j = 0; j < MAX_JOBS; ++j) if (!c->jobs[j].used) { slot = j; break; }
         if (slot < 0) return OMC_OK;
@@ -714,7 +740,7 @@ static int admit(omc_dev_ctx *c)
         for (auto &kv : c->pool) pooled += (int64_t)kv.first;
         if (c->budget > 0 && c->used - pooled + (int64_t)work_bytes(w) > c->budget) {
             if (any_running(c)) return OMC_OK;          // wait for memory to free up
-            c->waiting.pop_front();
+            c->waiting.erase(c->waiting.begin() + at);
             fail(OMC_EBUDGET, "request needs %zu bytes, budget leaves %lld", work_bytes(w),
                  (long long)(c->budget - (c->used - pooled)));
             end_work(c, w, OMC_EBUDGET);
@@ -722,57 +748,75 @@ static int admit(omc_dev_ctx *c)
         }
         JobRec &J = c->jobs[slot];
         J = JobRec();
-        J.tally_bytes = sizeof(double) * (size_t)w->nreg * w->slots;
+        const bool split = w->split_k > 0;
+        const int64_t jn = split ? w->nperbatch : w->nhist;      // histories of this job
+        const int slots = split ? 1 : w->slots;
+        J.tally_bytes = sizeof(double) * (size_t)w->nreg * slots;
         int rc = OMC_OK;
-        if (w->scratch) {
+        if (split && !w->acc) {
+            rc = dalloc(c, (void **)&w->acc, 2 * sizeof(double) * (size_t)w->nreg);
+            if (!rc) launch_zero_f64(w->acc, 2 * w->nreg, c->aux);
+        }
+        if (!rc && w->scratch) {
             // the caller checked that both fit (omc_dev_run_dosxyz_noalloc)
             J.borrowed = true;
             J.tally = (double *)c->scratch;
             J.cdf = (double *)((char *)c->scratch + ((J.tally_bytes + 255) & ~(size_t)255));
-        } else {
+        } else if (!rc) {
             rc = pool_alloc(c, (void **)&J.tally, J.tally_bytes);
             if (!rc) rc = dalloc(c, (void **)&J.cdf, 2 * OMC_INVDIM * sizeof(double));
         }
         if (!rc && w->ledger) {
-            J.ledger_bytes = sizeof(double) * OMC_LEDGER * (size_t)w->nhist;
+            J.ledger_bytes = sizeof(double) * OMC_LEDGER * (size_t)jn;
             rc = dalloc(c, (void **)&J.ledger, J.ledger_bytes);
         }
         if (rc) {
             J.used = true;
             release_job(c, slot);
             if (any_running(c)) return OMC_OK;
-            c->waiting.pop_front();
+            c->waiting.erase(c->waiting.begin() + at);
             end_work(c, w, rc);
             continue;
         }
         J.used = true;
         J.w = w;
+        J.nhist = jn;
+        J.batch = split ? w->next_batch : -1;
         // serving requests share each refill in slices of chunk/8 (spreading
         // a 6.5e5-history request over all six pipelines instead raised the
         // FP32 p50 of the latency stream from 57 to 69-77 ms); a synchronous
         // run spreads its histories over all pipelines
         const int64_t npipes = (int64_t)c->pipes.size();
         J.quantum = w->ticket >= 0 ? std::max<int64_t>(4096, c->chunk / 8)
-                                   : std::max<int64_t>(65536, (w->nhist + npipes - 1) / npipes);
-        launch_zero_f64(J.tally, w->nreg * w->slots, c->aux);
-        if (J.ledger) launch_zero_f64(J.ledger, OMC_LEDGER * w->nhist, c->aux);
+                                   : std::max<int64_t>(65536, (jn + npipes - 1) / npipes);
+        launch_zero_f64(J.tally, w->nreg * slots, c->aux);
+        if (J.ledger) launch_zero_f64(J.ledger, OMC_LEDGER * jn, c->aux);
         HIPCHK(hipMemcpyAsync(J.cdf, w->cdf.data(), 2 * OMC_INVDIM * sizeof(double), hipMemcpyHostToDevice, c->aux));
         DevJob d = w->tmpl;
         d.phantom = w->phantom;
         d.cdf1 = J.cdf;
         d.cdf2 = J.cdf + OMC_INVDIM;
-        // one grid per batch; a job that keeps one grid puts every history in it
-        d.nperbatch = w->slots == w->nbatch ? w->nperbatch : w->nhist;
+        // one grid per batch; a job that keeps one grid puts every history in
+        // it.  A batch admitted on its own is a job of its histories: their
+        // streams start at the batch's first history, as in the whole work.
+        d.nperbatch = slots == w->nbatch ? w->nperbatch : jn;
+        if (split) d.hist0 = w->tmpl.hist0 + (uint64_t)J.batch * (uint64_t)w->nperbatch;
         d.tally = J.tally;
         d.ledger = J.ledger;
         HIPCHK(hipMemcpyAsync(c->d_jobs + slot, &d, sizeof d, hipMemcpyHostToDevice, c->aux));
         HIPCHK(hipStreamSynchronize(c->aux));
         c->jobs_hi = std::max(c->jobs_hi, slot + 1);
         w->job = slot;
-        w->state = W_RUNNING;
-        w->admit_ms = now_ms(c);
-        c->waiting.pop_front();
+        if (w->state == W_WAITING) {
+            w->state = W_RUNNING;
+            w->admit_ms = now_ms(c);
+        }
         c->runnable.push_back(slot);
+        if (split) {
+            w->batches_live++;
+            if (++w->next_batch < w->nbatch) continue;   // the work stays in line
+        }
+        c->waiting.erase(c->waiting.begin() + at);
     }
     return OMC_OK;
 }
@@ -798,7 +842,17 @@ static int finish_fold(omc_dev_ctx *c, int j)
     const RunSink *k = w->sink;
     const int64_t nreg = w->nreg;
     hipStream_t s = c->aux;
-    if (wants_sums(w)) {
+    if (J.batch >= 0) {
+        // one batch of a work admitted batch by batch: into the work's sums;
+        // after its last batch the sums go to pinned memory
+        if (w->acc) {
+            launch_accum(J.tally, nreg, w->acc, w->acc + nreg, s);
+            if (w->batches_done + 1 == w->nbatch &&
+                hipMemcpyAsync(c->h_sums, w->acc, 2 * (size_t)nreg * sizeof(double), hipMemcpyDeviceToHost, s) !=
+                    hipSuccess)
+                return fail(OMC_EDEVICE, "copy tallies");
+        }
+    } else if (wants_sums(w)) {
         launch_fold(J.tally, w->nbatch, nreg, c->fin_sums, c->fin_sums + nreg, s);
         if (hipMemcpyAsync(c->h_sums, c->fin_sums, 2 * (size_t)nreg * sizeof(double), hipMemcpyDeviceToHost, s) !=
             hipSuccess)
@@ -841,7 +895,8 @@ static int finish_deliver(omc_dev_ctx *c, int j)
     Work *w = J.w;
     RunSink *k = w->sink;
     const int64_t nreg = w->nreg;
-    if (wants_sums(w)) {
+    const bool last = J.batch < 0 || w->batches_done + 1 == w->nbatch;
+    if (last && wants_sums(w)) {
         const double *h = c->h_sums;
         if (!k) {
             w->sum.assign(h, h + nreg);
@@ -852,9 +907,9 @@ static int finish_deliver(omc_dev_ctx *c, int j)
             for (int64_t i = 0; i < nreg; ++i) { s1[i] += h[i]; s2[i] += h[nreg + i]; }
         }
     }
+    const int64_t first = w->ledger_off + (J.batch >= 0 ? (int64_t)J.batch * w->nperbatch : 0);
     if (k && k->ledger && J.ledger &&
-        hipMemcpy(k->ledger + OMC_LEDGER * w->ledger_off, J.ledger, J.ledger_bytes, hipMemcpyDeviceToHost) !=
-            hipSuccess)
+        hipMemcpy(k->ledger + OMC_LEDGER * first, J.ledger, J.ledger_bytes, hipMemcpyDeviceToHost) != hipSuccess)
         return fail(OMC_EDEVICE, "copy ledger");
     if (k && k->sparse) {
         k->rows[w->index].assign(c->h_rows, c->h_rows + J.nnz);
@@ -864,7 +919,7 @@ static int finish_deliver(omc_dev_ctx *c, int j)
 }
 
 // Advances the collection of finished jobs without blocking.  Returns the
-// number of jobs delivered, or an error (the failing job is ended with it);
+// number of jobs delivered (a batch of a work admitted batch by batch counts), or an error (the failing job is ended with it);
 // *worked tells whether any stage advanced (otherwise it only polled).
 static int pump_finish(omc_dev_ctx *c, bool *worked)
 {
@@ -887,9 +942,16 @@ static int pump_finish(omc_dev_ctx *c, bool *worked)
                 rc = finish_deliver(c, j);
                 if (!rc) {
                     Work *w = J.w;
+                    const bool batch = J.batch >= 0;
                     c->finishing.pop_front();
                     release_job(c, j);
-                    end_work(c, w, OMC_OK);
+                    if (batch) {
+                        w->batches_live--;
+                        w->batches_done++;
+                    }
+                    if (!batch || w->batches_done == w->nbatch) end_work(c, w, OMC_OK);
+                    // a finished batch frees memory for the work's next one:
+                    // the scheduler admits before it waits again
                     delivered++;
                 }
             }
@@ -930,7 +992,7 @@ static int pull_segments(omc_dev_ctx *c, int pi, int64_t room, uint32_t &injP, u
         int j = c->runnable.front();
         c->runnable.pop_front();
         JobRec &J = c->jobs[j];
-        int64_t left = J.w->nhist - J.injected;
+        int64_t left = J.nhist - J.injected;
         // A primary photon of a splitting job is followed as nsplit
         // sub-photons in one lane, each setting up to two charged particles
         // in motion, all pushed before the lane lets go of it: it counts as
@@ -953,7 +1015,7 @@ static int pull_segments(omc_dev_ctx *c, int pi, int64_t room, uint32_t &injP, u
             J.pipes |= 1u << pi;
             P.open.push_back(j);
         }
-        if (J.injected < J.w->nhist) c->runnable.push_back(j);
+        if (J.injected < J.nhist) c->runnable.push_back(j);
     }
     return nseg;
 }
@@ -1090,7 +1152,7 @@ static int complete_generation(omc_dev_ctx *c, int pi)
     P.open.swap(still);
     for (int j : check) {
         JobRec &J = c->jobs[j];
-        if (J.used && J.fin == FIN_NONE && J.pipes == 0 && J.injected == J.w->nhist) {
+        if (J.used && J.fin == FIN_NONE && J.pipes == 0 && J.injected == J.nhist) {
             J.fin = FIN_QUEUED;
             c->finishing.push_back(j);
         }
@@ -1229,10 +1291,33 @@ static void stats_end(omc_dev_ctx *c, omc_dev_stats *out, int64_t histories, dou
 
 // Runs the works of one synchronous call to completion (serving requests
 // already admitted keep running alongside).
+// A work whose whole batch tally would exceed the context's tally_bytes is
+// admitted batch by batch, with as many batches in flight as the limit
+// leaves room for beside its two sum grids, but at least two and at least
+// enough histories to keep half of the pipelines' refill room (pipelines x
+// chunk) busy: each batch holds its grid until its slowest particle is
+// gone, and with small batches in flight the pipelines would run mostly
+// such tails (C5 plan, 6e5 histories per batch, four batches per beam in
+// flight: 6.1e7 histories/s against 7.8e7 with whole tallies, 1036
+// generations against 105).  No split where it would not save a grid.
+static void plan_split(const omc_dev_ctx *c, Work &w)
+{
+    w.split_k = 0;
+    const int64_t grid = (int64_t)sizeof(double) * w.nreg;
+    if (c->tally_cap <= 0 || w.scratch || w.nbatch < 2 || w.slots != w.nbatch) return;
+    if (grid * w.nbatch <= c->tally_cap) return;
+    const int64_t inflight = (int64_t)c->pipes.size() * c->chunk / 2;
+    const int64_t k = std::max<int64_t>({ c->tally_cap / grid - 2, (inflight + w.nperbatch - 1) / w.nperbatch,
+                                         (int64_t)2 });
+    if (k + 2 >= w.nbatch) return;
+    w.split_k = (int)k;
+}
+
 static int run_sink(omc_dev_ctx *c, RunSink &k, std::vector<Work> &works, omc_dev_stats *stats)
 {
     k.remaining = (int)works.size();
     for (auto &w : works) {
+        plan_split(c, w);
         w.sink = &k;
         w.submit_ms = now_ms(c);
         c->waiting.push_back(&w);
@@ -1347,6 +1432,7 @@ int omc_dev_create(omc_dev_ctx **out, const omc_dev_opts *opts)
     if (hipGetDeviceProperties(&prop, o.device) == hipSuccess) c->cus = prop.multiProcessorCount;
     c->drain_min = o.drain_min > 0 ? o.drain_min : o.drain_min < 0 ? 0 : drain_min_default(c->cus, c->precision);
     c->woodcock = o.woodcock > 0 || (o.woodcock == 0 && o.precision == OMC_PREC_FP32);
+    c->tally_cap = o.tally_bytes > 0 ? o.tally_bytes : o.tally_bytes < 0 ? 0 : DEFAULT_TALLY_CAP;
     size_t freeb = 0, totalb = 0;
     (void)hipMemGetInfo(&freeb, &totalb);
     c->budget = o.hbm_budget > 0 ? o.hbm_budget : (int64_t)(0.9 * (double)freeb);
@@ -1380,7 +1466,10 @@ int omc_dev_destroy(omc_dev_ctx *c)
         DevBuf *bufs[] = { &P.xb, &P.yb, &P.zb, &P.med, &P.rhof, &P.rho, &P.wc };
         for (auto *b : bufs) if (b->p) (void)hipFree(b->p);
     }
-    for (auto &kv : c->tickets) delete kv.second;
+    for (auto &kv : c->tickets) {
+        if (kv.second->acc) (void)hipFree(kv.second->acc);
+        delete kv.second;
+    }
     if (c->aux) (void)hipStreamDestroy(c->aux);
     delete c;
     return OMC_OK;
@@ -1656,6 +1745,7 @@ static int tune_impl(omc_dev_ctx *c, const std::function<int()> &run, int64_t hi
     best->woodcock = c->woodcock ? 1 : -1;
     best->event_div = (int)c->event_div;
     best->fetch_min = (int)c->fetch_min;
+    best->tally_bytes = c->tally_cap > 0 ? c->tally_cap : -1;
     *hps = bv;
     return OMC_OK;
 }
@@ -1833,6 +1923,7 @@ int omc_dev_submit(omc_dev_ctx *c, const omc_dev_request *req)
     w->nreg = c->ph[req->phantom_id].nreg;
     w->ticket = c->next_ticket++;
     w->submit_ms = now_ms(c);
+    plan_split(c, *w);
     c->tickets[w->ticket] = w;
     c->waiting.push_back(w);
     c->unfinished++;

@@ -23,6 +23,18 @@ __global__ void __launch_bounds__(256) k_fold(const double *tally, int nbatch, i
     s2[i] = b;
 }
 
+// One finished batch of a job admitted batch by batch: its energies added to
+// the job's sums and their squares to the sums of squares (the same per-batch
+// terms as k_fold, in the order the batches end).
+__global__ void __launch_bounds__(256) k_accum(const double *grid, int64_t nreg, double *s, double *s2)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nreg) return;
+    const double e = grid[i];
+    s[i] += e;
+    s2[i] += e * e;
+}
+
 __global__ void k_set_u32(uint32_t *p, int n, uint32_t v)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -227,6 +239,12 @@ void launch_fold(const double *tally, int nbatch, int64_t nreg, double *s, doubl
 {
     if (nreg <= 0) return;
     hipLaunchKernelGGL(k_fold, dim3((unsigned)((nreg + 255) / 256)), dim3(256), 0, st, tally, nbatch, nreg, s, s2);
+}
+
+void launch_accum(const double *grid, int64_t nreg, double *s, double *s2, hipStream_t st)
+{
+    if (nreg <= 0) return;
+    hipLaunchKernelGGL(k_accum, dim3((unsigned)((nreg + 255) / 256)), dim3(256), 0, st, grid, nreg, s, s2);
 }
 
 void launch_set_u32(uint32_t *p, int n, uint32_t v, hipStream_t st)

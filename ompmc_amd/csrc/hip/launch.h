@@ -20,6 +20,7 @@ template <typename R> struct Launch {
 
 // precision-independent helpers (defined in k64.hip)
 void launch_fold(const double *tally, int nbatch, int64_t nreg, double *s, double *s2, hipStream_t st);
+void launch_accum(const double *grid, int64_t nreg, double *s, double *s2, hipStream_t st);
 void launch_set_u32(uint32_t *p, int n, uint32_t v, hipStream_t st);
 void launch_zero_f64(double *p, int64_t n, hipStream_t st);
 
