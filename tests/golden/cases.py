@@ -50,6 +50,10 @@ CASES = {
 # defined.  The device is compared with the oracle on them.
 DEVICE_CASES = {
     "pairAg52MeV": (silver_slab, dict(mono=52.0, field=(-1, 1, -1, 1)), 300),
+    # the C2 and C3 configurations at their full grids: the 64^3 slab at
+    # 0.25 cm and the 128x128x96 patient at 0.25 cm, 6 MV, 10x10 cm^2
+    "C2": (lambda: cfg.config_phantom("C2"), dict(spectrum=True, field=(-5, 5, -5, 5)), 2000),
+    "C3": (lambda: cfg.config_phantom("C3"), dict(spectrum=True, field=(-5, 5, -5, 5)), 1000),
 }
 
 

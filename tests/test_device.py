@@ -109,8 +109,8 @@ def assert_same_histories(r, what):
 
 
 REPLAY = ["water2MeV", "slab6MV", "patient6MV", "electron6MeV", "positron6MeV", "pencil", "oversize", "C1",
-          "pairAg8MeV", "pairAg52MeV"]
-REPLAY_NCASE = {"pairAg8MeV": 3000, "pairAg52MeV": 300}
+          "pairAg8MeV", "pairAg52MeV", "C2", "C3"]
+REPLAY_NCASE = {"pairAg8MeV": 3000, "pairAg52MeV": 300, "C2": 4000, "C3": 2000}
 
 
 @pytest.mark.parametrize("case", REPLAY)
@@ -120,9 +120,10 @@ def test_fp64_replays_oracle_histories(oracle, D, tmp_path, case):
     single scattering, charged-particle howfar call, photon and electron
     loop iteration, voxel crossing and deposit.  A wrong pair() or
     annihilation, an off-by-one table bin or voxel index changes a count or
-    a history.  C1 is the 40^3 water / 2 MeV configuration; pairAg* put
-    photons of 8 and 52 MeV into silver, where pair production, Bhabha
-    scattering and annihilation run hundreds of times."""
+    a history.  C1, C2 and C3 are those configurations at their full grids
+    (40^3 water / 2 MeV, the 64^3 slab and the 128x128x96 patient under
+    6 MV); pairAg* put photons of 8 and 52 MeV into silver, where pair
+    production, Bhabha scattering and annihilation run hundreds of times."""
     r = run_both(oracle, D, write_case(case, tmp_path, ncase=REPLAY_NCASE.get(case, 1000)))
     assert_same_histories(r, case)
     assert abs(r["sd"].sum() / r["so"].sum() - 1) < 1e-12
