@@ -298,17 +298,46 @@ struct SourceSeg {
 // Structure-of-arrays particle queue.  A particle that gave up its lane in
 // the middle of its transport (a yield) also carries the lane's state, so
 // the next launch resumes it exactly where it stopped.
+//
+// One allocation per queue: QR_N arrays of R (e x y z u v w wt, then the six
+// real-valued resume variables), then QU_N arrays of uint32 (ir, meta =
+// job << 3 | resume << 2 | (iq + 1), history index within the job, Philox
+// key and counter, packed integer resume variables, loop trips taken so
+// far), each `cap` long.  A queue is then three scalars -- base, tail
+// counter, capacity -- and a kernel's three queues fit its scalar
+// registers; with one pointer per array they took 135 of them and the
+// compiler spilled pointers into vector-register lanes, reading them back
+// before every table or queue access.
+enum { QR_E, QR_X, QR_Y, QR_Z, QR_U, QR_V, QR_W, QR_WT, QR_ST, QR_N = QR_ST + 6 };
+enum { QU_IR, QU_META, QU_HIST, QU_K0, QU_K1, QU_CTR, QU_SFLAGS, QU_TRIPS, QU_N };
+
 template <typename R> struct Queue {
-    R *e, *x, *y, *z, *u, *v, *w, *wt;
-    int32_t *ir;
-    uint32_t *meta;          // job << 3 | resume << 2 | (iq + 1)
-    uint32_t *hist;          // history index within the job
-    uint32_t *k0, *k1, *ctr;
-    R *st[6];                // resume state: real-valued lane variables
-    uint32_t *sflags;        // resume state: packed integer lane variables
-    uint32_t *trips;         // transport loop trips taken so far
+    char *base;
     uint32_t *count;         // device tail counter
     uint32_t cap;
+
+    __host__ __device__ R *r(int f) const { return reinterpret_cast<R *>(base) + (size_t)f * cap; }
+    __host__ __device__ uint32_t *u(int f) const
+    {
+        return reinterpret_cast<uint32_t *>(base + sizeof(R) * (size_t)QR_N * cap) + (size_t)f * cap;
+    }
+    __host__ __device__ R *e() const { return r(QR_E); }
+    __host__ __device__ R *x() const { return r(QR_X); }
+    __host__ __device__ R *y() const { return r(QR_Y); }
+    __host__ __device__ R *z() const { return r(QR_Z); }
+    __host__ __device__ R *uu() const { return r(QR_U); }
+    __host__ __device__ R *v() const { return r(QR_V); }
+    __host__ __device__ R *w() const { return r(QR_W); }
+    __host__ __device__ R *wt() const { return r(QR_WT); }
+    __host__ __device__ R *st(int k) const { return r(QR_ST + k); }
+    __host__ __device__ int32_t *ir() const { return reinterpret_cast<int32_t *>(u(QU_IR)); }
+    __host__ __device__ uint32_t *meta() const { return u(QU_META); }
+    __host__ __device__ uint32_t *hist() const { return u(QU_HIST); }
+    __host__ __device__ uint32_t *k0() const { return u(QU_K0); }
+    __host__ __device__ uint32_t *k1() const { return u(QU_K1); }
+    __host__ __device__ uint32_t *ctr() const { return u(QU_CTR); }
+    __host__ __device__ uint32_t *sflags() const { return u(QU_SFLAGS); }
+    __host__ __device__ uint32_t *trips() const { return u(QU_TRIPS); }
 };
 
 constexpr uint32_t META_RESUME = 4u;
@@ -324,14 +353,14 @@ template <typename R> struct Particle {
 template <typename R>
 __device__ __forceinline__ bool q_load(const Queue<R> &q, uint32_t i, Particle<R> &p)
 {
-    p.e = q.e[i]; p.x = q.x[i]; p.y = q.y[i]; p.z = q.z[i];
-    p.u = q.u[i]; p.v = q.v[i]; p.w = q.w[i]; p.wt = q.wt[i];
-    p.ir = q.ir[i];
-    uint32_t m = q.meta[i];
+    p.e = q.e()[i]; p.x = q.x()[i]; p.y = q.y()[i]; p.z = q.z()[i];
+    p.u = q.uu()[i]; p.v = q.v()[i]; p.w = q.w()[i]; p.wt = q.wt()[i];
+    p.ir = q.ir()[i];
+    uint32_t m = q.meta()[i];
     p.job = m >> 3;
     p.iq = (int)(m & 3u) - 1;
-    p.hist = q.hist[i];
-    p.rng.k0 = q.k0[i]; p.rng.k1 = q.k1[i]; p.rng.ctr = q.ctr[i];
+    p.hist = q.hist()[i];
+    p.rng.k0 = q.k0()[i]; p.rng.k1 = q.k1()[i]; p.rng.ctr = q.ctr()[i];
     p.rng.reset_cache();
     return (m & META_RESUME) != 0u;
 }
@@ -339,12 +368,12 @@ __device__ __forceinline__ bool q_load(const Queue<R> &q, uint32_t i, Particle<R
 template <typename R>
 __device__ __forceinline__ void q_store(const Queue<R> &q, uint32_t i, const Particle<R> &p, uint32_t resume = 0u)
 {
-    q.e[i] = p.e; q.x[i] = p.x; q.y[i] = p.y; q.z[i] = p.z;
-    q.u[i] = p.u; q.v[i] = p.v; q.w[i] = p.w; q.wt[i] = p.wt;
-    q.ir[i] = p.ir;
-    q.meta[i] = (p.job << 3) | resume | (uint32_t)(p.iq + 1);
-    q.hist[i] = p.hist;
-    q.k0[i] = p.rng.k0; q.k1[i] = p.rng.k1; q.ctr[i] = p.rng.ctr;
+    q.e()[i] = p.e; q.x()[i] = p.x; q.y()[i] = p.y; q.z()[i] = p.z;
+    q.uu()[i] = p.u; q.v()[i] = p.v; q.w()[i] = p.w; q.wt()[i] = p.wt;
+    q.ir()[i] = p.ir;
+    q.meta()[i] = (p.job << 3) | resume | (uint32_t)(p.iq + 1);
+    q.hist()[i] = p.hist;
+    q.k0()[i] = p.rng.k0; q.k1()[i] = p.rng.k1; q.ctr()[i] = p.rng.ctr;
 }
 
 // Physics events counted by the transport kernels, in the order of the

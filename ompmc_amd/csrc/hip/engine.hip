@@ -137,7 +137,7 @@ struct PipeMirror {
 
 struct Pipe {
     hipStream_t stream = nullptr;
-    void *q[4][21] = {};          // [photon cur/next, charged cur/next][field]
+    void *q[4] = {};              // particle queues: photon cur/next, charged cur/next
     uint32_t *d_counts = nullptr; // 4 tails + 2 fetch counters
     DevStatus *d_status = nullptr;
     uint32_t *d_flags = nullptr;  // [MAX_JOBS]
@@ -338,20 +338,12 @@ void pool_free(omc_dev_ctx *c, void *p, size_t bytes)
 }
 
 // per queue slot: 8 reals, ir/meta/hist/k0, k1+ctr, 6 resume reals, sflags, trips
-template <typename R> size_t particle_bytes() { return 14 * sizeof(R) + 8 * sizeof(uint32_t); }
+template <typename R> size_t particle_bytes() { return QR_N * sizeof(R) + QU_N * sizeof(uint32_t); }
 
 template <typename R> Queue<R> queue_view(const omc_dev_ctx *c, const Pipe &P, int k)
 {
     Queue<R> q;
-    void *const *a = P.q[k];
-    q.e = (R *)a[0]; q.x = (R *)a[1]; q.y = (R *)a[2]; q.z = (R *)a[3];
-    q.u = (R *)a[4]; q.v = (R *)a[5]; q.w = (R *)a[6]; q.wt = (R *)a[7];
-    q.ir = (int32_t *)a[8]; q.meta = (uint32_t *)a[9]; q.hist = (uint32_t *)a[10];
-    q.k0 = (uint32_t *)a[11]; q.k1 = (uint32_t *)a[12];
-    q.ctr = (uint32_t *)a[12] + c->qcap;     // k1 and ctr share one allocation
-    for (int f = 0; f < 6; ++f) q.st[f] = (R *)a[13 + f];
-    q.sflags = (uint32_t *)a[19];
-    q.trips = (uint32_t *)a[20];
+    q.base = (char *)P.q[k];
     q.count = P.d_counts + k;
     q.cap = c->qcap;
     return q;
@@ -363,13 +355,7 @@ template <typename R> int alloc_pipe(omc_dev_ctx *c, Pipe &P)
     hipEvent_t *evs[] = { &P.ev_src0, &P.ev_src1, &P.ev_ph0, &P.ev_ph1, &P.ev_el0, &P.ev_el1, &P.ev_begin,
                           &P.ev_end[0], &P.ev_end[1] };
     for (auto *e : evs) HIPCHK(hipEventCreate(e));
-    for (int k = 0; k < 4; ++k) {
-        for (int f = 0; f < 8; ++f) OMC_TRY_DEV(dalloc(c, &P.q[k][f], sizeof(R) * (size_t)c->qcap));
-        for (int f = 8; f < 12; ++f) OMC_TRY_DEV(dalloc(c, &P.q[k][f], 4 * (size_t)c->qcap));
-        OMC_TRY_DEV(dalloc(c, &P.q[k][12], 8 * (size_t)c->qcap));
-        for (int f = 13; f < 19; ++f) OMC_TRY_DEV(dalloc(c, &P.q[k][f], sizeof(R) * (size_t)c->qcap));
-        for (int f = 19; f < 21; ++f) OMC_TRY_DEV(dalloc(c, &P.q[k][f], 4 * (size_t)c->qcap));
-    }
+    for (int k = 0; k < 4; ++k) OMC_TRY_DEV(dalloc(c, &P.q[k], particle_bytes<R>() * (size_t)c->qcap));
     OMC_TRY_DEV(dalloc(c, (void **)&P.d_counts, 8 * sizeof(uint32_t)));
     OMC_TRY_DEV(dalloc(c, (void **)&P.d_status, sizeof(DevStatus)));
     OMC_TRY_DEV(dalloc(c, (void **)&P.d_flags, MAX_JOBS * sizeof(uint32_t)));
@@ -385,7 +371,7 @@ template <typename R> int alloc_pipe(omc_dev_ctx *c, Pipe &P)
 void free_pipe(Pipe &P)
 {
     if (P.stream) (void)hipStreamSynchronize(P.stream);
-    for (auto &qq : P.q) for (auto &p : qq) if (p) { (void)hipFree(p); p = nullptr; }
+    for (auto &p : P.q) if (p) { (void)hipFree(p); p = nullptr; }
     void *ptrs[] = { P.d_counts, P.d_status, P.d_flags, P.d_segs };
     for (void *p : ptrs) if (p) (void)hipFree(p);
     if (P.h) (void)hipHostFree(P.h);
@@ -1090,7 +1076,7 @@ template <typename R> static int launch_generation(omc_dev_ctx *c, int pi)
     const int njobs = std::max(c->jobs_hi, 1);
     launch_set_u32(P.d_flags, njobs, 0u, s);
     const Queue<R> qpn = queue_view<R>(c, P, nxt), qen = queue_view<R>(c, P, 2 + nxt);
-    launch_job_flags(qpn.meta, qpn.count, qen.meta, qen.count, c->qcap, P.d_flags, njobs, s);
+    launch_job_flags(qpn.meta(), qpn.count, qen.meta(), qen.count, c->qcap, P.d_flags, njobs, s);
     HIPCHK(hipMemcpyAsync(P.h->counts, P.d_counts, 8 * sizeof(uint32_t), hipMemcpyDeviceToHost, s));
     HIPCHK(hipMemcpyAsync(&P.h->status, P.d_status, sizeof(DevStatus), hipMemcpyDeviceToHost, s));
     HIPCHK(hipMemcpyAsync(P.h->flags, P.d_flags, njobs * sizeof(uint32_t), hipMemcpyDeviceToHost, s));

@@ -140,7 +140,7 @@ __device__ __forceinline__ void yield_push(const Queue<R> &q, const Particle<R> 
     if (i < q.cap) {
         q_store(q, i, p, META_RESUME);
         s.save(q, i);
-        q.trips[i] = trips;
+        q.trips()[i] = trips;
     } else {
         atomicOr(&st->overflow, 1u);
     }
@@ -172,14 +172,14 @@ template <typename R> struct PhotonLane {
 
     __device__ __forceinline__ void save(const Queue<R> &q, uint32_t i) const
     {
-        q.st[0][i] = gle; q.st[1][i] = dpmfp; q.st[2][i] = gmfp; q.st[3][i] = cohfac; q.st[4][i] = eig;
-        q.sflags[i] = pack_lane(phase, 0, imed, -1, lgle);
+        q.st(0)[i] = gle; q.st(1)[i] = dpmfp; q.st(2)[i] = gmfp; q.st(3)[i] = cohfac; q.st(4)[i] = eig;
+        q.sflags()[i] = pack_lane(phase, 0, imed, -1, lgle);
     }
     __device__ __forceinline__ void load(const Queue<R> &q, uint32_t i)
     {
-        gle = q.st[0][i]; dpmfp = q.st[1][i]; gmfp = q.st[2][i]; cohfac = q.st[3][i]; eig = q.st[4][i];
+        gle = q.st(0)[i]; dpmfp = q.st(1)[i]; gmfp = q.st(2)[i]; cohfac = q.st(3)[i]; eig = q.st(4)[i];
         int flag, medold;
-        unpack_lane(q.sflags[i], phase, flag, imed, medold, lgle);
+        unpack_lane(q.sflags()[i], phase, flag, imed, medold, lgle);
         split = 0u;
     }
 };
@@ -543,15 +543,15 @@ template <typename R> struct ElectronLane {
 
     __device__ __forceinline__ void save(const Queue<R> &q, uint32_t i) const
     {
-        q.st[0][i] = rhof; q.st[1][i] = eke; q.st[2][i] = elke;
-        q.st[3][i] = demfp; q.st[4][i] = sig0; q.st[5][i] = ustep;
-        q.sflags[i] = pack_lane(phase, compute_tstep, imed, medold, lelke);
+        q.st(0)[i] = rhof; q.st(1)[i] = eke; q.st(2)[i] = elke;
+        q.st(3)[i] = demfp; q.st(4)[i] = sig0; q.st(5)[i] = ustep;
+        q.sflags()[i] = pack_lane(phase, compute_tstep, imed, medold, lelke);
     }
     __device__ __forceinline__ void load(const Queue<R> &q, uint32_t i)
     {
-        rhof = q.st[0][i]; eke = q.st[1][i]; elke = q.st[2][i];
-        demfp = q.st[3][i]; sig0 = q.st[4][i]; ustep = q.st[5][i];
-        unpack_lane(q.sflags[i], phase, compute_tstep, imed, medold, lelke);
+        rhof = q.st(0)[i]; eke = q.st(1)[i]; elke = q.st(2)[i];
+        demfp = q.st(3)[i]; sig0 = q.st(4)[i]; ustep = q.st(5)[i];
+        unpack_lane(q.sflags()[i], phase, compute_tstep, imed, medold, lelke);
     }
 };
 
@@ -1074,7 +1074,7 @@ __global__ void __launch_bounds__(256, Waves<R>::photon) k_photon(KArgs<R> A)
             if (i < in_end) {
                 if (q_load(A.qin, i, p)) {
                     s.load(A.qin, i);
-                    trips = A.qin.trips[i];
+                    trips = A.qin.trips()[i];
                 } else {
                     s.phase = 0;
                     s.split = 0u;
@@ -1156,7 +1156,7 @@ __global__ void __launch_bounds__(256, Waves<R>::electron) k_electron(KArgs<R> A
             if (i < in_end) {
                 if (q_load(A.qin, i, p)) {
                     s.load(A.qin, i);
-                    trips = A.qin.trips[i];
+                    trips = A.qin.trips()[i];
                 } else {
                     s.phase = 0;
                     trips = 0;

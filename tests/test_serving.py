@@ -98,6 +98,22 @@ def test_admission_follows_hbm_budget(D, problems):
     assert r3.status == -9 and out[t3][1] is None
 
 
+def test_media_uploads_do_not_leak_budget(D, problems):
+    """Each media upload replaces the table buffers and returns their bytes
+    to the budget: many uploads under a tight budget still leave room for
+    a request (without that, the table bytes of every upload would add up
+    and admission would refuse it)."""
+    pp, _ = problems
+    nreg = pp.reg.nreg
+    with D.Engine(0, hbm_budget=256 << 20) as eng:
+        for _ in range(80):             # about 3 MB of tables each
+            eng.upload_media(pp.tab)
+        pid = eng.upload_phantom(pp.ph, pp.reg)
+        t = eng.submit(pid, pp.src, 4000, nbatch=10, seed=(9, 9))
+        r = drain(eng, [t], {t: nreg})[t][0]
+    assert r.status == 0 and r.nhist == 4000
+
+
 def test_request_errors(D, problems):
     pp, src = problems
     with D.Engine(0) as eng:
