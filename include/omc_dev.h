@@ -261,6 +261,17 @@ int  omc_dev_tune_beamlets(omc_dev_ctx *ctx, const omc_source_beamlets *src, con
 int  omc_dev_math_eval(int device, int precision, int fn, int64_t n, const double *x, const double *y,
                        double *out);
 
+/* initPhotonData on the GPU (src/ompmc.c:216-665, the part omc_build_photon
+ * restates on the host): the photon mean free path, branching ratios and
+ * Rayleigh ratio of every medium of t (gmfp0/1, gbr10/11, gbr20/21,
+ * cohe0/1, and ge0/ge1) from the XCOM element data in data_folder, written
+ * into t's arrays in omc_build_photon's layout.  t's pegs4 fields (ap, up,
+ * rho, ne, elements; omc_read_pegs) must be filled and its photon arrays
+ * allocated.  The device evaluates the host builder's arithmetic in the
+ * same order; an entry differs from the host's only where the device's exp
+ * or log rounds differently from glibc's (a few ulps at most). */
+int  omc_dev_build_photon(int device, omc_tables *t, const char *data_folder);
+
 /* ---- serving: continuous batching of dose requests ----------------------- */
 typedef struct {
     int      phantom_id;

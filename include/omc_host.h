@@ -157,6 +157,20 @@ void omc_tables_free(omc_tables *t);
  * distributed with ompMC).  Binary layout of src/ompmc.c:2401-2607. */
 int  omc_write_synthetic_spinms(const char *path);
 
+/* The four XCOM element data sets the photon tables are built from (photo
+ * absorption, Rayleigh, pair, triplet; read as getXsecData reads them,
+ * src/ompmc.c:287-345), flattened for a device builder: set s, element
+ * Z = z + 1 has n[s][z] (log E, log sigma) points starting at off[s][z] in
+ * le and ls. */
+typedef struct {
+    int     n[4][OMC_MXELEMENT];
+    int64_t off[4][OMC_MXELEMENT];
+    int64_t total;
+    double *le, *ls;
+} omc_xcom_data;
+int  omc_xcom_load(omc_xcom_data *x, const char *data_folder);
+void omc_xcom_free(omc_xcom_data *x);
+
 /* Individual pieces, exposed for tests. */
 double omc_kn_sigma0(double e);
 void   omc_spline_setup(const double *x, const double *f, double *a, double *b,

@@ -1862,6 +1862,91 @@ int omc_dev_tune_beamlets(omc_dev_ctx *c, const omc_source_beamlets *src, const 
     }, hist, best, hps);
 }
 
+int omc_dev_build_photon(int device, omc_tables *t, const char *data_folder)
+{
+    if (!t || !data_folder) return fail(OMC_EARG, "null argument");
+    const int nmed = t->nmed;
+    if (nmed < 1 || nmed > OMC_MXMED) return fail(OMC_ERANGE, "nmed %d", nmed);
+    double *outs[8] = { t->gmfp1, t->gmfp0, t->gbr11, t->gbr10, t->gbr21, t->gbr20, t->cohe1, t->cohe0 };
+    for (double *p : outs) if (!p) return fail(OMC_EARG, "photon arrays not allocated");
+    // per-medium scalars and the element order, as the host builder forms them
+    std::vector<PhotonBuildMedium> meds((size_t)nmed);
+    for (int i = 0; i < nmed; ++i) {
+        PhotonBuildMedium &m = meds[(size_t)i];
+        memset(&m, 0, sizeof m);
+        const int ne = t->ne[i];
+        if (ne < 1 || ne > OMC_MXELEMENT) return fail(OMC_ERANGE, "medium %d: %d elements", i, ne);
+        t->ge1[i] = (double)(OMC_MXGE - 1) / log(t->up[i] / t->ap[i]);
+        t->ge0[i] = 1.0 - t->ge1[i] * log(t->ap[i]);
+        double sum_a = 0.0, sum_z = 0.0;
+        for (int j = 0; j < ne; ++j) {
+            sum_a += t->elements[i][j].pz * t->elements[i][j].wa;
+            sum_z += t->elements[i][j].pz * t->elements[i][j].z;
+        }
+        int idx[OMC_MXELEMENT];
+        for (int j = 0; j < ne; ++j) idx[j] = j;
+        for (int j = 1; j < ne; ++j) {           // ascending Z (distinct Z: one order)
+            const int k = idx[j];
+            int q = j - 1;
+            while (q >= 0 && t->elements[i][idx[q]].z > t->elements[i][k].z) { idx[q + 1] = idx[q]; --q; }
+            idx[q + 1] = k;
+        }
+        for (int j = 0; j < ne; ++j) {
+            m.z[j] = t->elements[i][idx[j]].z;
+            m.pz[j] = t->elements[i][idx[j]].pz;
+            const int zi = (int)(m.z[j] + 0.5);
+            if (zi < 1 || zi > OMC_MXELEMENT)
+                return fail(OMC_ERANGE, "medium %d: element Z = %d outside the XCOM data (1..%d)", i, zi, OMC_MXELEMENT);
+        }
+        m.ne = ne;
+        m.ge0 = t->ge0[i];
+        m.ge1 = t->ge1[i];
+        m.sum_z = sum_z;
+        m.con2 = t->rho[i] / (sum_a * 1.6605655);
+    }
+    omc_xcom_data x;
+    if (omc_xcom_load(&x, data_folder) != OMC_OK) return fail(OMC_EIO, "%s", omc_last_error());
+    HIPCHK(hipSetDevice(device));
+    const size_t G = (size_t)nmed * OMC_MXGE;
+    const size_t bytes_meds = sizeof(PhotonBuildMedium) * (size_t)nmed;
+    const size_t bytes_n = sizeof(int) * 4 * OMC_MXELEMENT, bytes_off = sizeof(int64_t) * 4 * OMC_MXELEMENT;
+    const size_t bytes_pts = sizeof(double) * (size_t)x.total;
+    char *d = nullptr;
+    const size_t total = bytes_meds + bytes_n + bytes_off + 2 * bytes_pts + sizeof(double) * (4 * G + 8 * G) + 6 * 256;
+    int rc = OMC_OK;
+    if (hipMalloc((void **)&d, total) != hipSuccess) {
+        omc_xcom_free(&x);
+        return fail(OMC_EDEVICE, "hipMalloc(%zu) for the photon tables", total);
+    }
+    auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    char *p = d;
+    PhotonBuildMedium *d_meds = (PhotonBuildMedium *)p; p += up(bytes_meds);
+    int *d_n = (int *)p;                                 p += up(bytes_n);
+    int64_t *d_off = (int64_t *)p;                       p += up(bytes_off);
+    double *d_le = (double *)p;                          p += up(bytes_pts);
+    double *d_ls = (double *)p;                          p += up(bytes_pts);
+    double *d_nodes = (double *)p;                       p += up(sizeof(double) * 4 * G);
+    double *d_tab = (double *)p;
+    if (hipMemcpy(d_meds, meds.data(), bytes_meds, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d_n, x.n, bytes_n, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d_off, x.off, bytes_off, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d_le, x.le, bytes_pts, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(d_ls, x.ls, bytes_pts, hipMemcpyHostToDevice) != hipSuccess)
+        rc = fail(OMC_EDEVICE, "copy the XCOM data");
+    if (!rc) {
+        XcomView xv{ d_n, d_off, d_le, d_ls };
+        launch_photon_build(d_meds, nmed, xv, d_nodes, d_tab, nullptr);
+        if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess)
+            rc = fail(OMC_EDEVICE, "photon table kernels");
+    }
+    for (int q = 0; !rc && q < 8; ++q)
+        if (hipMemcpy(outs[q], d_tab + (size_t)q * G, sizeof(double) * G, hipMemcpyDeviceToHost) != hipSuccess)
+            rc = fail(OMC_EDEVICE, "copy the photon tables");
+    (void)hipFree(d);
+    omc_xcom_free(&x);
+    return rc;
+}
+
 int omc_dev_math_eval(int device, int precision, int fn, int64_t n, const double *x, const double *y, double *out)
 {
     if (!x || !out || n < 0 || fn < 0 || fn > 5) return fail(OMC_EARG, "bad argument");

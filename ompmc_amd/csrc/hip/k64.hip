@@ -23,6 +23,128 @@ __global__ void __launch_bounds__(256) k_fold(const double *tally, int nbatch, i
     s2[i] = b;
 }
 
+// ---- photon tables on the device (initPhotonData, src/ompmc.c:216-665) ----
+// The host builder's arithmetic in its order (FP64, no contraction): a
+// table entry differs from the host's only where the device's exp or log
+// rounds a value differently from glibc's.
+
+// Klein-Nishina total cross section per electron (src/ompmc.c:451-469).
+__device__ double kn_sigma0(double e)
+{
+    const double con = 0.1274783851;
+    const double ko = e / RM;
+    if (ko < 0.01) return 8.0 * con / 3.0 * (1.0 - ko * (2.0 - ko * (5.2 - 13.3 * ko))) / RM;
+    const double c1 = 1.0 / (ko * ko);
+    const double c2 = 1.0 - 2.0 * (1.0 + ko) * c1;
+    const double c3 = (1.0 + 2.0 * ko) * c1;
+    const double eps2 = 1.0;
+    const double eps1 = 1.0 / (1.0 + 2.0 * ko);
+    return (c1 * (1.0 / eps1 - 1.0 / eps2) + c2 * log(eps2 / eps1) + eps2 * (c3 + 0.5 * eps2) -
+            eps1 * (c3 + 0.5 * eps1)) / e * con;
+}
+
+// Partial cross section of a medium at log energy gle (get_data,
+// src/ompmc.c:347-449): each element's data interpolated log-log, summed in
+// ascending Z weighted by the atom fractions.  Pair (kind 1) and triplet
+// (kind 2) data get a threshold point at eth = 2m / 4m, log sigma corrected
+// by -3 log(1 - eth/E) at every point, and the factor (1 - eth/E)^3.
+__device__ double medium_sigma(int kind, const PhotonBuildMedium &m, const XcomView &x, int set, double gle,
+                               double e)
+{
+    double res = 0.0;
+    for (int i = 0; i < m.ne; ++i) {
+        const int z = (int)(m.z[i] + 0.5) - 1;
+        const int n0 = x.n[set * OMC_MXELEMENT + z];
+        const double *le = x.le + x.off[set * OMC_MXELEMENT + z];
+        const double *ls = x.ls + x.off[set * OMC_MXELEMENT + z];
+        double sig = 0.0;
+        if (kind == 0) {
+            if (!((gle < le[0]) || (gle >= le[n0 - 1]))) {
+                int k;
+                for (k = 0; k < n0 - 1; ++k)
+                    if ((gle >= le[k]) && (gle < le[k + 1])) break;
+                const double p = (gle - le[k]) / (le[k + 1] - le[k]);
+                sig = exp(p * ls[k + 1] + (1.0 - p) * ls[k]);
+            }
+        } else {
+            const double eth = kind == 1 ? 2.0 * RM : 4.0 * RM;
+            const int n = n0 + 1;
+            // point j of the extended set: j = 0 at the threshold (with the
+            // corrected log sigma of the first data point), j >= 1 data point j - 1
+            const double le_th = log(eth);
+            auto lej = [&](int j) { return j == 0 ? le_th : le[j - 1]; };
+            auto lsj = [&](int j) {
+                const int q = j == 0 ? 0 : j - 1;
+                return ls[q] - 3.0 * log(1.0 - eth / exp(le[q]));
+            };
+            if ((gle < lej(0)) || (gle >= lej(n - 1))) {
+                sig = (gle < lej(0)) ? 0.0 : exp(lsj(n - 1));
+            } else {
+                int k;
+                for (k = 0; k < n - 1; ++k)
+                    if ((gle >= lej(k)) && (gle < lej(k + 1))) break;
+                const double p = (gle - lej(k)) / (lej(k + 1) - lej(k));
+                sig = exp(p * lsj(k + 1) + (1.0 - p) * lsj(k));
+            }
+            if (e > eth) sig *= (1.0 - eth / e) * (1.0 - eth / e) * (1.0 - eth / e);
+        }
+        res += m.pz[i] * sig;
+    }
+    return res;
+}
+
+// Node values at the grid energies, one block per medium.
+__global__ void __launch_bounds__(256) k_photon_nodes(const PhotonBuildMedium *meds, XcomView x, double *nodes)
+{
+    const PhotonBuildMedium &m = meds[blockIdx.x];
+    double *out = nodes + (size_t)blockIdx.x * 4 * MXGE;
+    for (int j = threadIdx.x; j < MXGE; j += blockDim.x) {
+        const double gle = ((double)(j + 1) - m.ge0) / m.ge1;
+        const double e = exp(gle);
+        const double s_photo = medium_sigma(0, m, x, 0, gle, e);
+        const double s_rayl = medium_sigma(0, m, x, 1, gle, e);
+        const double s_pair = medium_sigma(1, m, x, 2, gle, e);
+        const double s_trip = medium_sigma(2, m, x, 3, gle, e);
+        const double sig_kn = m.sum_z * kn_sigma0(e);
+        const double sig_p = s_pair + s_trip;
+        const double sigma = sig_kn + sig_p + s_photo;
+        const double gbr1 = sig_p / sigma;
+        out[j] = 1.0 / (sigma * m.con2);
+        out[MXGE + j] = gbr1;
+        out[2 * MXGE + j] = gbr1 + sig_kn / sigma;
+        out[3 * MXGE + j] = sigma / (s_rayl + sigma);
+    }
+}
+
+// Linear fits in log energy: bin b from nodes b and b + 1 at node b + 1's
+// log energy; the last bin keeps the slope of the one before it.
+__global__ void __launch_bounds__(256) k_photon_bins(const PhotonBuildMedium *meds, int nmed, const double *nodes,
+                                                     double *tab)
+{
+    const int m = blockIdx.x;
+    const double ge0 = meds[m].ge0, ge1 = meds[m].ge1;
+    const double *nd = nodes + (size_t)m * 4 * MXGE;
+    const size_t plane = (size_t)nmed * MXGE;
+    for (int b = threadIdx.x; b < MXGE; b += blockDim.x) {
+        const int j = b < MXGE - 1 ? b + 1 : MXGE - 1;
+        const double gle = ((double)(j + 1) - ge0) / ge1;
+        const size_t ix = (size_t)m * MXGE + b;
+        for (int q = 0; q < 4; ++q) {
+            const double v = nd[q * MXGE + j];
+            const double slope = (v - nd[q * MXGE + j - 1]) * ge1;
+            tab[(2 * q) * plane + ix] = slope;
+            tab[(2 * q + 1) * plane + ix] = v - slope * gle;
+        }
+    }
+}
+
+void launch_photon_build(const PhotonBuildMedium *meds, int nmed, const XcomView &x, double *nodes, double *tab,
+                         hipStream_t st)
+{
+    hipLaunchKernelGGL(k_photon_nodes, dim3(nmed), dim3(256), 0, st, meds, x, nodes);
+    hipLaunchKernelGGL(k_photon_bins, dim3(nmed), dim3(256), 0, st, meds, nmed, (const double *)nodes, tab);
+}
+
 // One finished batch of a job admitted batch by batch: its energies added to
 // the job's sums and their squares to the sums of squares (the same per-batch
 // terms as k_fold, in the order the batches end).

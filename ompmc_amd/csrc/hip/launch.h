@@ -18,6 +18,31 @@ template <typename R> struct Launch {
     static void math_eval(int fn, int64_t n, const double *x, const double *y, double *out, hipStream_t s);
 };
 
+// One medium for the photon table builder (initPhotonData): its elements in
+// ascending Z with their atom fractions, and the per-medium scalars the
+// host computes in the host builder's order (log-energy grid ge0/ge1, the
+// sum of Z and the conversion to 1/cm).
+struct PhotonBuildMedium {
+    int ne;
+    int pad;
+    double z[OMC_MXELEMENT], pz[OMC_MXELEMENT];
+    double ge0, ge1, sum_z, con2;
+};
+
+// XCOM element data on the device: set s (photo, Rayleigh, pair, triplet),
+// element Z = z + 1 has n[s*OMC_MXELEMENT + z] points at off[...] in le, ls.
+struct XcomView {
+    const int *n;
+    const int64_t *off;
+    const double *le, *ls;
+};
+
+// Photon tables of nmed media: node values [nmed][4][MXGE] (gmfp, gbr1,
+// gbr2, cohe at the grid energies), then the per-bin fits into
+// tab [8][nmed*MXGE] (gmfp1, gmfp0, gbr11, gbr10, gbr21, gbr20, cohe1, cohe0).
+void launch_photon_build(const PhotonBuildMedium *meds, int nmed, const XcomView &x, double *nodes, double *tab,
+                         hipStream_t st);
+
 // precision-independent helpers (defined in k64.hip)
 void launch_fold(const double *tally, int nbatch, int64_t nreg, double *s, double *s2, hipStream_t st);
 void launch_accum(const double *grid, int64_t nreg, double *s, double *s2, hipStream_t st);

@@ -57,6 +57,45 @@ bad:
     return omc_set_error(OMC_EFORMAT, "Could not read the data file %s", path);
 }
 
+int omc_xcom_load(omc_xcom_data *x, const char *data_folder)
+{
+    static const char *names[4] = { "xcom_photo.data", "xcom_rayleigh.data", "xcom_pair.data",
+                                    "xcom_triplet.data" };
+    if (!x || !data_folder) return omc_set_error(OMC_EARG, "null argument");
+    memset(x, 0, sizeof *x);
+    xcom_set s[4];
+    memset(s, 0, sizeof s);
+    int rc = OMC_OK;
+    for (int k = 0; k < 4 && !rc; ++k) rc = xcom_read(&s[k], data_folder, names[k]);
+    if (!rc) {
+        for (int k = 0; k < 4; ++k)
+            for (int z = 0; z < OMC_MXELEMENT; ++z) {
+                x->n[k][z] = s[k].n[z];
+                x->off[k][z] = x->total;
+                x->total += s[k].n[z];
+            }
+        x->le = malloc((size_t)x->total * sizeof(double));
+        x->ls = malloc((size_t)x->total * sizeof(double));
+        if (!x->le || !x->ls) rc = omc_set_error(OMC_ENOMEM, "XCOM data");
+    }
+    for (int k = 0; !rc && k < 4; ++k)
+        for (int z = 0; z < OMC_MXELEMENT; ++z) {
+            memcpy(x->le + x->off[k][z], s[k].le[z], (size_t)s[k].n[z] * sizeof(double));
+            memcpy(x->ls + x->off[k][z], s[k].ls[z], (size_t)s[k].n[z] * sizeof(double));
+        }
+    for (int k = 0; k < 4; ++k) xcom_free(&s[k]);
+    if (rc) omc_xcom_free(x);
+    return rc;
+}
+
+void omc_xcom_free(omc_xcom_data *x)
+{
+    if (!x) return;
+    free(x->le);
+    free(x->ls);
+    memset(x, 0, sizeof *x);
+}
+
 /* Element order used when summing partial cross sections: ascending Z.
  * (The reference heap-sorts; for a medium's distinct Z values every correct
  * sort gives this same order, which fixes the floating-point sum order.) */

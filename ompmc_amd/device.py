@@ -84,6 +84,7 @@ def lib():
         L.omc_dev_tune.argtypes = [C.c_void_p, C.POINTER(N.SourceDosxyz), C.POINTER(RunParams), C.c_int64,
                                    C.POINTER(DevOpts), C.POINTER(C.c_double)]
         L.omc_dev_math_eval.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int64, _dp, _dp, _dp]
+        L.omc_dev_build_photon.argtypes = [C.c_int, C.POINTER(N.Tables), C.c_char_p]
         L.omc_dev_tune_beamlets.argtypes = [C.c_void_p, C.POINTER(N.SourceBeamlets), C.POINTER(C.c_int64),
                                             C.POINTER(RunParams), C.POINTER(DevOpts), C.POINTER(C.c_double)]
         _i64p = C.POINTER(C.c_int64)
@@ -103,6 +104,16 @@ def check(rc):
     if rc < 0:
         raise RuntimeError(f"omc_dev error {rc}: {lib().omc_dev_last_error().decode()}")
     return rc
+
+
+def build_photon(tables, data_folder, device=0):
+    """omc_dev_build_photon: the photon tables of `tables` (an N.Tables with
+    its pegs4 fields filled and its photon arrays allocated), computed on
+    the GPU from the XCOM data in data_folder."""
+    folder = str(data_folder)
+    if not folder.endswith("/"):
+        folder += "/"
+    check(lib().omc_dev_build_photon(device, C.byref(tables), folder.encode()))
 
 
 class Engine:

@@ -141,3 +141,34 @@ def test_photon_tables_physics_invariants(host):
         assert 13.5 < mfp < 14.8
     finally:
         host.omc_tables_free(C.byref(t))
+
+
+def test_xcom_flat_loader_matches_files(host):
+    """omc_xcom_load (the device photon builder's input) holds each XCOM file's
+    points in file order: counts and values as written, element by element."""
+    class Xcom(C.Structure):
+        _fields_ = [("n", (C.c_int * 50) * 4), ("off", (C.c_int64 * 50) * 4), ("total", C.c_int64),
+                    ("le", C.POINTER(C.c_double)), ("ls", C.POINTER(C.c_double))]
+    host.omc_xcom_load.argtypes = [C.POINTER(Xcom), C.c_char_p]
+    host.omc_xcom_free.argtypes = [C.POINTER(Xcom)]
+    x = Xcom()
+    folder = N.default_paths().data_folder
+    N.check(host.omc_xcom_load(C.byref(x), folder))
+    try:
+        le = np.ctypeslib.as_array(x.le, shape=(x.total,))
+        ls = np.ctypeslib.as_array(x.ls, shape=(x.total,))
+        for s, name in enumerate(("xcom_photo", "xcom_rayleigh", "xcom_pair", "xcom_triplet")):
+            tok = (Path(folder.decode()) / f"{name}.data").read_text().split()
+            pos = 0
+            for z in range(50):
+                n = int(tok[pos])
+                pos += 1
+                vals = np.array(tok[pos:pos + 2 * n], float).reshape(n, 2)
+                pos += 2 * n
+                o = x.off[s][z]
+                assert x.n[s][z] == n
+                assert np.array_equal(le[o:o + n], vals[:, 0]) and np.array_equal(ls[o:o + n], vals[:, 1])
+                assert np.all(np.diff(le[o:o + n]) >= 0)
+        assert x.total == sum(x.n[s][z] for s in range(4) for z in range(50))
+    finally:
+        host.omc_xcom_free(C.byref(x))
