@@ -564,8 +564,10 @@ def c4_section(workdir, nreq=40, load=0.8, seed=4242):
         return src
 
 
+    rhos = [density(p) for p in probs]
+
     def run(prec, reqs, rate_note):
-        lat, hist = [], 0
+        lat, hist, sig = [], 0, []
         with D.Engine(prec) as eng:
             eng.upload_media(probs[2].tab)
             pids = [eng.upload_phantom(p.ph, p.reg) for p in probs]
@@ -581,12 +583,14 @@ def c4_section(workdir, nreq=40, load=0.8, seed=4242):
                     i += 1
                 left = eng.step() if open_t else 0
                 for tk in list(open_t):
-                    res, _, _ = eng.poll(tk)
+                    ph = reqs[open_t[tk]].phantom
+                    res, s, s2 = eng.poll(tk, probs[ph].reg.nreg)
                     if res.done:
                         if res.status != 0:
                             raise RuntimeError(f"request {open_t[tk]} failed: {res.status}")
                         lat.append(res.finish_ms - res.submit_ms)
                         hist += res.nhist
+                        sig.append(sigma_at_dmax(s, s2, 10, rhos[ph]))
                         del open_t[tk]
                 if not left and not open_t and i < len(reqs):
                     time.sleep(max(0.0, min(0.002, reqs[i].arrival_s - (time.perf_counter() - t0))))
@@ -594,7 +598,9 @@ def c4_section(workdir, nreq=40, load=0.8, seed=4242):
         lat = np.array(lat)
         return {"requests": len(lat), "p50_ms": float(np.percentile(lat, 50)),
                 "p99_ms": float(np.percentile(lat, 99)), "mean_ms": float(lat.mean()),
-                "histories_per_s": hist / el, "seconds": el}
+                "histories_per_s": hist / el, "seconds": el,
+                "sigma_at_dmax_median": float(np.median(sig)), "sigma_at_dmax_max": float(max(sig)),
+                "requests_at_or_below_2pct_sigma": int(sum(x <= 0.02 for x in sig))}
 
     # capacity: FP64 histories/s on a closed batch of the stream's requests
     probe = cfg.serving_stream(nreq, 1.0, seed=seed)
