@@ -66,7 +66,8 @@ if int(os.environ.get("GPU_MAX_HW_QUEUES", "0") or 0) < 8:
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "histories/s and p50/p99 beamlet-dose latency at 2% σ, FP32-batched vs FP64 GPU"
-TUNE_KEYS = ("pipelines", "chunk", "yield_trips", "drain_lanes", "drain_trips", "event_div", "fetch_min")
+TUNE_KEYS = ("pipelines", "chunk", "yield_trips", "drain_lanes", "drain_trips", "event_div", "fetch_min",
+             "sort_min")
 
 
 def log(msg):

@@ -97,6 +97,16 @@ typedef struct {
                                 where that saves no grid.  Results equal the whole-tally ones to
                                 FP64 summation order.  0 = 512 MiB, < 0 =
                                 never split */
+    int64_t sort_min;        /* charged-particle launches whose input holds at
+                                least this many particles take them in order
+                                of falling energy (a counting sort of the
+                                queue's indices into 64 log-energy buckets
+                                before the launch): lanes of a wavefront
+                                start on similar particles and the longest
+                                tracks start first.  The histories do not
+                                depend on it.  0 = automatic (launches
+                                larger than the lanes the device holds
+                                resident), < 0 = never */
 } omc_dev_opts;
 
 /* Device counters of one run (all sums over the run). */

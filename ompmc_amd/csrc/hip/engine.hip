@@ -111,6 +111,11 @@ constexpr uint32_t DEFAULT_EVENT_DIV = 2;
 // 128x128x96 phantom take 126 MB); a C5-sized job (84 MB per grid) is
 // admitted batch by batch, four batches in flight beside its sum grids.
 constexpr int64_t DEFAULT_TALLY_CAP = (int64_t)512 << 20;
+// Energy-ordered fetch of charged-particle launches whose input exceeds the
+// lanes the device holds resident (drain_min_default): only then does the
+// order in which particles start matter.  Slab, 5e7 histories: FP32 1.80e8
+// -> 1.97e8 histories/s (charged-particle kernel time -15%), FP64 9.5e7 ->
+// 1.03e8; the histories are the same.
 constexpr uint32_t DEFAULT_FETCH_MIN = 1;
 
 struct PhantomSlot {
@@ -142,6 +147,8 @@ struct Pipe {
     DevStatus *d_status = nullptr;
     uint32_t *d_flags = nullptr;  // [MAX_JOBS]
     SourceSeg *d_segs = nullptr;  // [MAX_SEGS]
+    uint32_t *d_perm = nullptr;   // [qcap] energy-ordered fetch of a charged-particle launch
+    uint32_t *d_sort = nullptr;   // [SORT_BUCKETS * SORT_BLOCKS] its histograms / offsets
     PipeMirror *h = nullptr;
     hipEvent_t ev_src0 = nullptr, ev_src1 = nullptr, ev_ph0 = nullptr, ev_ph1 = nullptr;
     hipEvent_t ev_el0 = nullptr, ev_el1 = nullptr, ev_begin = nullptr;
@@ -235,6 +242,7 @@ struct omc_dev_ctx {
     int64_t used = 0;
     void *scratch = nullptr;          // this device's scratch array (device address)
     int64_t tally_cap = 0;            // largest whole batch tally of a job (bytes; 0 = no limit)
+    int64_t sort_min = 0;             // charged-particle launches of at least this many are energy-ordered (0 = never)
     int64_t chunk = 0;
     uint32_t qcap = 0;
     uint32_t yield_trips = 0;
@@ -360,6 +368,8 @@ template <typename R> int alloc_pipe(omc_dev_ctx *c, Pipe &P)
     OMC_TRY_DEV(dalloc(c, (void **)&P.d_status, sizeof(DevStatus)));
     OMC_TRY_DEV(dalloc(c, (void **)&P.d_flags, MAX_JOBS * sizeof(uint32_t)));
     OMC_TRY_DEV(dalloc(c, (void **)&P.d_segs, MAX_SEGS * sizeof(SourceSeg)));
+    OMC_TRY_DEV(dalloc(c, (void **)&P.d_perm, sizeof(uint32_t) * (size_t)c->qcap));
+    OMC_TRY_DEV(dalloc(c, (void **)&P.d_sort, sizeof(uint32_t) * SORT_BUCKETS * SORT_BLOCKS));
     HIPCHK(hipHostMalloc((void **)&P.h, sizeof(PipeMirror), hipHostMallocDefault));
     memset(P.h, 0, sizeof(PipeMirror));
     HIPCHK(hipMemset(P.d_counts, 0, 8 * sizeof(uint32_t)));
@@ -372,7 +382,7 @@ void free_pipe(Pipe &P)
 {
     if (P.stream) (void)hipStreamSynchronize(P.stream);
     for (auto &p : P.q) if (p) { (void)hipFree(p); p = nullptr; }
-    void *ptrs[] = { P.d_counts, P.d_status, P.d_flags, P.d_segs };
+    void *ptrs[] = { P.d_counts, P.d_status, P.d_flags, P.d_segs, P.d_perm, P.d_sort };
     for (void *p : ptrs) if (p) (void)hipFree(p);
     if (P.h) (void)hipHostFree(P.h);
     hipEvent_t evs[] = { P.ev_src0, P.ev_src1, P.ev_ph0, P.ev_ph1, P.ev_el0, P.ev_el1, P.ev_begin,
@@ -1046,6 +1056,7 @@ template <typename R> static int launch_generation(omc_dev_ctx *c, int pi)
     A.event_div = c->event_div;
     A.fetch_min = c->fetch_min;
     A.woodcock = c->woodcock ? 1u : 0u;
+    A.perm = nullptr;
     const int max_grid = c->cus * 8;
     P.had_ph = inP > 0;
     P.in_e = inE;
@@ -1069,6 +1080,12 @@ template <typename R> static int launch_generation(omc_dev_ctx *c, int pi)
         A.fetch = P.d_counts + 5;
         A.longest = P.d_counts + 6;
         HIPCHK(hipEventRecord(P.ev_el0, s));
+        // the input's final count is on the device once the photon kernel is
+        // done; the order is sorted there, inside the timed launch
+        if (c->sort_min > 0 && (int64_t)inE + 2 * (int64_t)inP >= c->sort_min) {
+            Launch<R>::sort_charged(qec, P.d_sort, P.d_perm, s);
+            A.perm = P.d_perm;
+        }
         Launch<R>::electron(A, (int)std::min<int64_t>(max_grid, ((int64_t)inE + 2 * (int64_t)inP + 255) / 256), s);
         HIPCHK(hipEventRecord(P.ev_el1, s));
     }
@@ -1355,7 +1372,7 @@ static int64_t pipe_bytes(const omc_dev_ctx *c)
 {
     const size_t pb = c->precision == OMC_PREC_FP64 ? particle_bytes<double>() : particle_bytes<float>();
     return 4 * (int64_t)pb * c->qcap + 8 * sizeof(uint32_t) + sizeof(DevStatus) + MAX_JOBS * sizeof(uint32_t) +
-           MAX_SEGS * sizeof(SourceSeg);
+           MAX_SEGS * sizeof(SourceSeg) + sizeof(uint32_t) * ((int64_t)c->qcap + SORT_BUCKETS * SORT_BLOCKS);
 }
 
 // (Re)builds the pipelines: npipes streams, each with queues four times
@@ -1419,6 +1436,7 @@ int omc_dev_create(omc_dev_ctx **out, const omc_dev_opts *opts)
     c->drain_min = o.drain_min > 0 ? o.drain_min : o.drain_min < 0 ? 0 : drain_min_default(c->cus, c->precision);
     c->woodcock = o.woodcock > 0 || (o.woodcock == 0 && o.precision == OMC_PREC_FP32);
     c->tally_cap = o.tally_bytes > 0 ? o.tally_bytes : o.tally_bytes < 0 ? 0 : DEFAULT_TALLY_CAP;
+    c->sort_min = o.sort_min > 0 ? o.sort_min : o.sort_min < 0 ? 0 : drain_min_default(c->cus, c->precision);
     size_t freeb = 0, totalb = 0;
     (void)hipMemGetInfo(&freeb, &totalb);
     c->budget = o.hbm_budget > 0 ? o.hbm_budget : (int64_t)(0.9 * (double)freeb);
@@ -1637,12 +1655,13 @@ struct TuneSetting {
     int pipes;
     int64_t chunk;
     int yield_trips, drain_lanes, drain_trips, event_div, fetch_min;
+    int64_t sort_min;
 };
 
 TuneSetting current_setting(const omc_dev_ctx *c)
 {
     return { (int)c->pipes.size(), c->chunk, (int)c->yield_trips, (int)c->drain_lanes, (int)c->drain_trips,
-             (int)c->event_div, (int)c->fetch_min };
+             (int)c->event_div, (int)c->fetch_min, c->sort_min };
 }
 
 int apply_setting(omc_dev_ctx *c, const TuneSetting &t)
@@ -1653,6 +1672,7 @@ int apply_setting(omc_dev_ctx *c, const TuneSetting &t)
     c->drain_trips = (uint32_t)t.drain_trips;
     c->event_div = (uint32_t)t.event_div;
     c->fetch_min = (uint32_t)t.fetch_min;
+    c->sort_min = t.sort_min;
     return OMC_OK;
 }
 
@@ -1681,7 +1701,7 @@ static int tune_impl(omc_dev_ctx *c, const std::function<int()> &run, int64_t hi
         }
         return OMC_OK;
     };
-    static const std::vector<int64_t> axes[7] = {
+    const std::vector<int64_t> axes[8] = {
         { 2, 4, 6, 8 },                              // pipelines
         { 1 << 20, 1 << 21, 1 << 22, 1 << 23 },      // live particles per pipeline
         { 1024, 4096, 16384 },                       // yield budget (trips)
@@ -1689,18 +1709,22 @@ static int tune_impl(omc_dev_ctx *c, const std::function<int()> &run, int64_t hi
         { 8, 16, 32 },                               // drain trips
         { 1, 2, 4 },                                 // photon event threshold divisor
         { 1, 4, 8, 16 },                             // fetch threshold (lanes)
+        { 0, drain_min_default(c->cus, c->precision) },   // energy-ordered charged launches (0 = never)
     };
     TuneSetting bst = current_setting(c);
     double bv = 0;
     int rc = measure(bst, bv);
-    for (int a = 0; a < 7 && !rc; ++a) {
+    for (int a = 0; a < 8 && !rc; ++a) {
         for (int64_t val : axes[a]) {
             TuneSetting t = bst;
-            int *fields[7] = { &t.pipes, nullptr, &t.yield_trips, &t.drain_lanes, &t.drain_trips, &t.event_div,
-                               &t.fetch_min };
+            int *fields[8] = { &t.pipes, nullptr, &t.yield_trips, &t.drain_lanes, &t.drain_trips, &t.event_div,
+                               &t.fetch_min, nullptr };
             if (a == 1) {
                 if (t.chunk == val) continue;
                 t.chunk = val;
+            } else if (a == 7) {
+                if (t.sort_min == val) continue;
+                t.sort_min = val;
             } else {
                 if (*fields[a] == (int)val) continue;
                 *fields[a] = (int)val;
@@ -1732,6 +1756,7 @@ static int tune_impl(omc_dev_ctx *c, const std::function<int()> &run, int64_t hi
     best->event_div = (int)c->event_div;
     best->fetch_min = (int)c->fetch_min;
     best->tally_bytes = c->tally_cap > 0 ? c->tally_cap : -1;
+    best->sort_min = c->sort_min > 0 ? c->sort_min : -1;
     *hps = bv;
     return OMC_OK;
 }

@@ -167,4 +167,93 @@ void Launch<R>::math_eval(int fn, int64_t n, const double *x, const double *y, d
     hipLaunchKernelGGL(k_math_eval<R>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, fn, n, x, y, out);
 }
 
+// ---- energy-ordered fetch of a charged-particle launch ---------------------
+// A counting sort of the queue's indices into SORT_BUCKETS buckets of total
+// energy, highest first, so that the lanes of a wavefront start on particles
+// of similar energy (their table lookups fall on few cache lines, their
+// steps are alike) and the longest tracks start first.  Stable within a
+// block's share; the histories do not depend on the order.  The queue's
+// final count is read on the device: the photon kernel before it appends.
+
+// Bucket of total energy e: 12 per octave from 0.5 MeV (below every
+// charged-particle cut-off) up, highest energy in bucket 0.
+template <typename R> __device__ __forceinline__ int sort_bucket(R e)
+{
+    const float l = (log2f((float)e) + 1.0f) * 12.0f;
+    int b = l > 0.0f ? (int)l : 0;
+    b = b > SORT_BUCKETS - 1 ? SORT_BUCKETS - 1 : b;
+    return SORT_BUCKETS - 1 - b;
+}
+
+// Share of the queue each of the SORT_BLOCKS blocks takes.
+__device__ __forceinline__ void sort_share(const uint32_t *count, uint32_t cap, uint32_t &lo, uint32_t &hi)
+{
+    const uint32_t n = min(*count, cap);
+    const uint32_t per = (n + SORT_BLOCKS - 1) / SORT_BLOCKS;
+    lo = min(blockIdx.x * per, n);
+    hi = min(lo + per, n);
+}
+
+// Per-block histograms, bucket-major: hist[b * SORT_BLOCKS + block].
+template <typename R>
+__global__ void __launch_bounds__(256) k_sort_count(Queue<R> q, uint32_t *hist)
+{
+    __shared__ uint32_t h[SORT_BUCKETS];
+    for (int b = threadIdx.x; b < SORT_BUCKETS; b += blockDim.x) h[b] = 0u;
+    __syncthreads();
+    uint32_t lo, hi;
+    sort_share(q.count, q.cap, lo, hi);
+    const R *e = q.e();
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += blockDim.x) atomicAdd(&h[sort_bucket<R>(e[i])], 1u);
+    __syncthreads();
+    for (int b = threadIdx.x; b < SORT_BUCKETS; b += blockDim.x) hist[b * SORT_BLOCKS + blockIdx.x] = h[b];
+}
+
+// Exclusive prefix sum of the SORT_BUCKETS * SORT_BLOCKS counts, in place
+// (one block of 1024 threads, each summing a run of the array).
+template <typename R> __global__ void __launch_bounds__(1024) k_sort_scan(uint32_t *hist)
+{
+    constexpr int N = SORT_BUCKETS * SORT_BLOCKS, RUN = N / 1024;
+    static_assert(N % 1024 == 0, "sort table size");
+    __shared__ uint32_t part[1024];
+    const int t = threadIdx.x;
+    uint32_t s = 0;
+    for (int k = 0; k < RUN; ++k) s += hist[t * RUN + k];
+    part[t] = s;
+    __syncthreads();
+    for (int d = 1; d < 1024; d <<= 1) {          // inclusive scan of the run sums
+        const uint32_t v = t >= d ? part[t - d] : 0u;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    uint32_t run = t ? part[t - 1] : 0u;
+    for (int k = 0; k < RUN; ++k) {
+        const uint32_t c = hist[t * RUN + k];
+        hist[t * RUN + k] = run;
+        run += c;
+    }
+}
+
+// Each block writes its share's indices at its offsets.
+template <typename R>
+__global__ void __launch_bounds__(256) k_sort_scatter(Queue<R> q, const uint32_t *offsets, uint32_t *perm)
+{
+    __shared__ uint32_t base[SORT_BUCKETS];
+    for (int b = threadIdx.x; b < SORT_BUCKETS; b += blockDim.x) base[b] = offsets[b * SORT_BLOCKS + blockIdx.x];
+    __syncthreads();
+    uint32_t lo, hi;
+    sort_share(q.count, q.cap, lo, hi);
+    const R *e = q.e();
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += blockDim.x) perm[atomicAdd(&base[sort_bucket<R>(e[i])], 1u)] = i;
+}
+
+template <typename R>
+void Launch<R>::sort_charged(const Queue<R> &q, uint32_t *hist, uint32_t *perm, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_sort_count<R>, dim3(SORT_BLOCKS), dim3(256), 0, s, q, hist);
+    hipLaunchKernelGGL(k_sort_scan<R>, dim3(1), dim3(1024), 0, s, hist);
+    hipLaunchKernelGGL(k_sort_scatter<R>, dim3(SORT_BLOCKS), dim3(256), 0, s, q, (const uint32_t *)hist, perm);
+}
+
 }  // namespace omc

@@ -21,7 +21,8 @@ class DevOpts(C.Structure):
     _fields_ = [("device", C.c_int), ("precision", C.c_int), ("hbm_budget", C.c_int64), ("chunk", C.c_int64),
                 ("pipelines", C.c_int), ("yield_trips", C.c_int), ("drain_lanes", C.c_int),
                 ("drain_trips", C.c_int), ("event_div", C.c_int), ("fetch_min", C.c_int),
-                ("drain_min", C.c_int64), ("woodcock", C.c_int), ("tally_bytes", C.c_int64)]
+                ("drain_min", C.c_int64), ("woodcock", C.c_int), ("tally_bytes", C.c_int64),
+                ("sort_min", C.c_int64)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -121,12 +122,12 @@ class Engine:
 
     def __init__(self, precision=FP64, device=0, hbm_budget=0, chunk=0, pipelines=0, yield_trips=0,
                  drain_lanes=0, drain_trips=0, event_div=0, fetch_min=0, drain_min=0, woodcock=0,
-                 tally_bytes=0):
+                 tally_bytes=0, sort_min=0):
         L = lib()
         self.ctx = C.c_void_p()
         self.nreg = {}
         opts = DevOpts(device, precision, hbm_budget, chunk, pipelines, yield_trips, drain_lanes, drain_trips,
-                       event_div, fetch_min, drain_min, woodcock, tally_bytes)
+                       event_div, fetch_min, drain_min, woodcock, tally_bytes, sort_min)
         check(L.omc_dev_create(C.byref(self.ctx), C.byref(opts)))
         self.precision = precision
 

@@ -26,7 +26,7 @@ def D():
     return device
 
 
-def run_both(oracle, D, stem, seed=(21, 5), precision=0, chunk=0):
+def run_both(oracle, D, stem, seed=(21, 5), precision=0, chunk=0, **engine):
     prob = oracle.load_problem(stem)
     try:
         so, so2, cnt = oracle.run_dosxyz_philox(prob, seed=seed, nthreads=1)
@@ -40,7 +40,7 @@ def run_both(oracle, D, stem, seed=(21, 5), precision=0, chunk=0):
         assert L.ora_run_dosxyz(ctx, C.byref(prob.src), prob.nperbatch, prob.nbatch, t1.ctypes.data_as(P),
                                 t2.ctypes.data_as(P)) == 0
         L.ora_destroy(ctx)
-        with D.Engine(precision, chunk=chunk) as eng:
+        with D.Engine(precision, chunk=chunk, **engine) as eng:
             eng.upload_media(prob.tab)
             pid = eng.upload_phantom(prob.ph, prob.reg)
             sd, sd2, st, led_d = eng.run_dosxyz(pid, prob.src, prob.nperbatch, prob.nbatch, seed=seed,
@@ -269,6 +269,15 @@ def test_schedule_and_reruns_do_not_change_histories(oracle, D, tmp_path, case, 
     for s, led, _, _ in out[1:]:
         assert np.allclose(s, out[0][0], rtol=1e-12, atol=1e-12 * out[0][0].max())
         assert np.allclose(led, out[0][1], rtol=1e-12, atol=1e-12)
+
+
+@pytest.mark.parametrize("case", ["slab6MV", "positron6MeV"])
+def test_energy_ordered_fetch_keeps_histories(oracle, D, tmp_path, case):
+    """Charged-particle launches taking their input in order of falling
+    energy (sort_min=1: every launch) run the same histories: each one's
+    ledger row, every event count and every voxel equal the oracle's."""
+    r = run_both(oracle, D, write_case(case, tmp_path, ncase=1000), sort_min=1)
+    assert_same_histories(r, case + " sorted")
 
 
 @pytest.mark.parametrize("case", ["electron6MeV", "slab6MV"])
