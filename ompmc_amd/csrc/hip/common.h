@@ -344,7 +344,7 @@ constexpr uint32_t META_RESUME = 4u;
 
 // energy-ordered fetch of charged-particle launches (kernels_impl.h)
 constexpr int SORT_BUCKETS = 64;
-constexpr int SORT_BLOCKS = 512;
+constexpr int SORT_BLOCKS = 128;
 
 template <typename R> struct Particle {
     R e, x, y, z, u, v, w, wt;

@@ -209,25 +209,23 @@ __global__ void __launch_bounds__(256) k_sort_count(Queue<R> q, uint32_t *hist)
     for (int b = threadIdx.x; b < SORT_BUCKETS; b += blockDim.x) hist[b * SORT_BLOCKS + blockIdx.x] = h[b];
 }
 
-// Exclusive prefix sum of the SORT_BUCKETS * SORT_BLOCKS counts, in place
-// (one block of 1024 threads, each summing a run of the array).
-template <typename R> __global__ void __launch_bounds__(1024) k_sort_scan(uint32_t *hist)
+// Exclusive prefix sum of the SORT_BUCKETS * SORT_BLOCKS counts, in place,
+// by one wavefront (each lane sums a run of the array).  One wavefront fits
+// beside the transport kernels of the other pipelines; a block of 1024
+// threads waited for a whole CU to empty, 6 ms per launch on average.
+template <typename R> __global__ void __launch_bounds__(64) k_sort_scan(uint32_t *hist)
 {
-    constexpr int N = SORT_BUCKETS * SORT_BLOCKS, RUN = N / 1024;
-    static_assert(N % 1024 == 0, "sort table size");
-    __shared__ uint32_t part[1024];
+    constexpr int N = SORT_BUCKETS * SORT_BLOCKS, RUN = N / 64;
+    static_assert(N % 64 == 0, "sort table size");
     const int t = threadIdx.x;
     uint32_t s = 0;
     for (int k = 0; k < RUN; ++k) s += hist[t * RUN + k];
-    part[t] = s;
-    __syncthreads();
-    for (int d = 1; d < 1024; d <<= 1) {          // inclusive scan of the run sums
-        const uint32_t v = t >= d ? part[t - d] : 0u;
-        __syncthreads();
-        part[t] += v;
-        __syncthreads();
+    uint32_t incl = s;                              // inclusive scan of the run sums
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t v = (uint32_t)__shfl_up((int)incl, d);
+        if (t >= d) incl += v;
     }
-    uint32_t run = t ? part[t - 1] : 0u;
+    uint32_t run = incl - s;
     for (int k = 0; k < RUN; ++k) {
         const uint32_t c = hist[t * RUN + k];
         hist[t * RUN + k] = run;
@@ -252,7 +250,7 @@ template <typename R>
 void Launch<R>::sort_charged(const Queue<R> &q, uint32_t *hist, uint32_t *perm, hipStream_t s)
 {
     hipLaunchKernelGGL(k_sort_count<R>, dim3(SORT_BLOCKS), dim3(256), 0, s, q, hist);
-    hipLaunchKernelGGL(k_sort_scan<R>, dim3(1), dim3(1024), 0, s, hist);
+    hipLaunchKernelGGL(k_sort_scan<R>, dim3(1), dim3(64), 0, s, hist);
     hipLaunchKernelGGL(k_sort_scatter<R>, dim3(SORT_BLOCKS), dim3(256), 0, s, q, (const uint32_t *)hist, perm);
 }
 
