@@ -100,7 +100,7 @@ typedef struct {
     int64_t sort_min;        /* charged-particle launches whose input holds at
                                 least this many particles take them in order
                                 of falling energy (a counting sort of the
-                                queue's indices into 64 log-energy buckets
+                                queue's records into 64 log-energy buckets
                                 before the launch): lanes of a wavefront
                                 start on similar particles and the longest
                                 tracks start first.  The histories do not

@@ -147,7 +147,7 @@ struct Pipe {
     DevStatus *d_status = nullptr;
     uint32_t *d_flags = nullptr;  // [MAX_JOBS]
     SourceSeg *d_segs = nullptr;  // [MAX_SEGS]
-    uint32_t *d_perm = nullptr;   // [qcap] energy-ordered fetch of a charged-particle launch
+    void *d_sorted = nullptr;     // [qcap] records: a charged-particle launch's input by falling energy
     uint32_t *d_sort = nullptr;   // [SORT_BUCKETS * SORT_BLOCKS] its histograms / offsets
     PipeMirror *h = nullptr;
     hipEvent_t ev_src0 = nullptr, ev_src1 = nullptr, ev_ph0 = nullptr, ev_ph1 = nullptr;
@@ -368,7 +368,7 @@ template <typename R> int alloc_pipe(omc_dev_ctx *c, Pipe &P)
     OMC_TRY_DEV(dalloc(c, (void **)&P.d_status, sizeof(DevStatus)));
     OMC_TRY_DEV(dalloc(c, (void **)&P.d_flags, MAX_JOBS * sizeof(uint32_t)));
     OMC_TRY_DEV(dalloc(c, (void **)&P.d_segs, MAX_SEGS * sizeof(SourceSeg)));
-    OMC_TRY_DEV(dalloc(c, (void **)&P.d_perm, sizeof(uint32_t) * (size_t)c->qcap));
+    OMC_TRY_DEV(dalloc(c, &P.d_sorted, particle_bytes<R>() * (size_t)c->qcap));
     OMC_TRY_DEV(dalloc(c, (void **)&P.d_sort, sizeof(uint32_t) * SORT_BUCKETS * SORT_BLOCKS));
     HIPCHK(hipHostMalloc((void **)&P.h, sizeof(PipeMirror), hipHostMallocDefault));
     memset(P.h, 0, sizeof(PipeMirror));
@@ -382,7 +382,7 @@ void free_pipe(Pipe &P)
 {
     if (P.stream) (void)hipStreamSynchronize(P.stream);
     for (auto &p : P.q) if (p) { (void)hipFree(p); p = nullptr; }
-    void *ptrs[] = { P.d_counts, P.d_status, P.d_flags, P.d_segs, P.d_perm, P.d_sort };
+    void *ptrs[] = { P.d_counts, P.d_status, P.d_flags, P.d_segs, P.d_sorted, P.d_sort };
     for (void *p : ptrs) if (p) (void)hipFree(p);
     if (P.h) (void)hipHostFree(P.h);
     hipEvent_t evs[] = { P.ev_src0, P.ev_src1, P.ev_ph0, P.ev_ph1, P.ev_el0, P.ev_el1, P.ev_begin,
@@ -1056,7 +1056,6 @@ template <typename R> static int launch_generation(omc_dev_ctx *c, int pi)
     A.event_div = c->event_div;
     A.fetch_min = c->fetch_min;
     A.woodcock = c->woodcock ? 1u : 0u;
-    A.perm = nullptr;
     const int max_grid = c->cus * 8;
     P.had_ph = inP > 0;
     P.in_e = inE;
@@ -1083,8 +1082,10 @@ template <typename R> static int launch_generation(omc_dev_ctx *c, int pi)
         // the input's final count is on the device once the photon kernel is
         // done; the order is sorted there, inside the timed launch
         if (c->sort_min > 0 && (int64_t)inE + 2 * (int64_t)inP >= c->sort_min) {
-            Launch<R>::sort_charged(qec, P.d_sort, P.d_perm, s);
-            A.perm = P.d_perm;
+            Queue<R> sorted = qec;
+            sorted.base = (char *)P.d_sorted;
+            Launch<R>::sort_charged(qec, P.d_sort, sorted, s);
+            A.qin = sorted;
         }
         Launch<R>::electron(A, (int)std::min<int64_t>(max_grid, ((int64_t)inE + 2 * (int64_t)inP + 255) / 256), s);
         HIPCHK(hipEventRecord(P.ev_el1, s));
@@ -1371,13 +1372,14 @@ const char *omc_dev_last_error(void) { return g_err; }
 static int64_t pipe_bytes(const omc_dev_ctx *c)
 {
     const size_t pb = c->precision == OMC_PREC_FP64 ? particle_bytes<double>() : particle_bytes<float>();
-    return 4 * (int64_t)pb * c->qcap + 8 * sizeof(uint32_t) + sizeof(DevStatus) + MAX_JOBS * sizeof(uint32_t) +
-           MAX_SEGS * sizeof(SourceSeg) + sizeof(uint32_t) * ((int64_t)c->qcap + SORT_BUCKETS * SORT_BLOCKS);
+    return 5 * (int64_t)pb * c->qcap + 8 * sizeof(uint32_t) + sizeof(DevStatus) + MAX_JOBS * sizeof(uint32_t) +
+           MAX_SEGS * sizeof(SourceSeg) + sizeof(uint32_t) * SORT_BUCKETS * SORT_BLOCKS;
 }
 
 // (Re)builds the pipelines: npipes streams, each with queues four times
-// `chunk` deep (a generation's secondaries); the queues take at most half
-// the budget, which can lower the chunk.  No work may be queued.
+// `chunk` deep (a generation's secondaries) and the sorted copy of a
+// charged-particle input; the five take at most half the budget, which can
+// lower the chunk.  No work may be queued.
 static int configure_pipes(omc_dev_ctx *c, int npipes, int64_t chunk)
 {
     for (auto &P : c->pipes) free_pipe(P);
@@ -1387,7 +1389,7 @@ static int configure_pipes(omc_dev_ctx *c, int npipes, int64_t chunk)
     const size_t pb = c->precision == OMC_PREC_FP64 ? particle_bytes<double>() : particle_bytes<float>();
     c->chunk = chunk;
     int64_t cap = std::max<int64_t>(4 * c->chunk, 1 << 12);
-    const int64_t cap_mem = c->budget / 2 / npipes / 4 / (int64_t)pb;
+    const int64_t cap_mem = c->budget / 2 / npipes / 5 / (int64_t)pb;
     if (cap > cap_mem) {
         cap = cap_mem;
         c->chunk = std::max<int64_t>(cap / 4, 1);

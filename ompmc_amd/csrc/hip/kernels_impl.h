@@ -233,9 +233,14 @@ template <typename R> __global__ void __launch_bounds__(64) k_sort_scan(uint32_t
     }
 }
 
-// Each block writes its share's indices at its offsets.
+// Each block copies its share's records to its offsets in the sorted queue
+// (the resume state only for records that carry one, a particle that
+// yielded).  The transport kernel then reads the sorted queue in order: its
+// fetches stay coalesced (reading through an index instead scatters every
+// lane's loads of a wavefront over as many cache lines, one or two records
+// used in each).
 template <typename R>
-__global__ void __launch_bounds__(256) k_sort_scatter(Queue<R> q, const uint32_t *offsets, uint32_t *perm)
+__global__ void __launch_bounds__(256) k_sort_scatter(Queue<R> q, const uint32_t *offsets, Queue<R> out)
 {
     __shared__ uint32_t base[SORT_BUCKETS];
     for (int b = threadIdx.x; b < SORT_BUCKETS; b += blockDim.x) base[b] = offsets[b * SORT_BLOCKS + blockIdx.x];
@@ -243,15 +248,27 @@ __global__ void __launch_bounds__(256) k_sort_scatter(Queue<R> q, const uint32_t
     uint32_t lo, hi;
     sort_share(q.count, q.cap, lo, hi);
     const R *e = q.e();
-    for (uint32_t i = lo + threadIdx.x; i < hi; i += blockDim.x) perm[atomicAdd(&base[sort_bucket<R>(e[i])], 1u)] = i;
+    for (uint32_t i = lo + threadIdx.x; i < hi; i += blockDim.x) {
+        const uint32_t j = atomicAdd(&base[sort_bucket<R>(e[i])], 1u);
+#pragma unroll
+        for (int f = 0; f < QR_ST; ++f) out.r(f)[j] = q.r(f)[i];
+#pragma unroll
+        for (int f = 0; f < QU_SFLAGS; ++f) out.u(f)[j] = q.u(f)[i];
+        if (q.meta()[i] & META_RESUME) {
+#pragma unroll
+            for (int f = QR_ST; f < QR_N; ++f) out.r(f)[j] = q.r(f)[i];
+#pragma unroll
+            for (int f = QU_SFLAGS; f < QU_N; ++f) out.u(f)[j] = q.u(f)[i];
+        }
+    }
 }
 
 template <typename R>
-void Launch<R>::sort_charged(const Queue<R> &q, uint32_t *hist, uint32_t *perm, hipStream_t s)
+void Launch<R>::sort_charged(const Queue<R> &q, uint32_t *hist, const Queue<R> &out, hipStream_t s)
 {
     hipLaunchKernelGGL(k_sort_count<R>, dim3(SORT_BLOCKS), dim3(256), 0, s, q, hist);
     hipLaunchKernelGGL(k_sort_scan<R>, dim3(1), dim3(64), 0, s, hist);
-    hipLaunchKernelGGL(k_sort_scatter<R>, dim3(SORT_BLOCKS), dim3(256), 0, s, q, (const uint32_t *)hist, perm);
+    hipLaunchKernelGGL(k_sort_scatter<R>, dim3(SORT_BLOCKS), dim3(256), 0, s, q, (const uint32_t *)hist, out);
 }
 
 }  // namespace omc

@@ -16,9 +16,9 @@ template <typename R> struct Launch {
     static void photon(const KArgs<R> &a, int grid, hipStream_t s);
     static void electron(const KArgs<R> &a, int grid, hipStream_t s);
     static void math_eval(int fn, int64_t n, const double *x, const double *y, double *out, hipStream_t s);
-    // perm[0, count) = the queue's indices, highest total energy first
+    // out[0, count) = the queue's records, highest total energy first
     // (hist: SORT_BUCKETS * SORT_BLOCKS words of scratch)
-    static void sort_charged(const Queue<R> &q, uint32_t *hist, uint32_t *perm, hipStream_t s);
+    static void sort_charged(const Queue<R> &q, uint32_t *hist, const Queue<R> &out, hipStream_t s);
 };
 
 // One medium for the photon table builder (initPhotonData): its elements in

@@ -40,7 +40,6 @@ template <typename R> struct KArgs {
     uint32_t fetch_min;      // free lanes take new particles once this many are free
     uint32_t woodcock;       // photon flights by delta tracking where the phantom allows
     uint32_t *longest;       // trips of the longest track that ended in this launch (atomicMax)
-    const uint32_t *perm;    // fetch order of the input (null: queue order)
 };
 
 // Per-lane geometry / scoring context for the particle the lane holds.  The
@@ -1155,7 +1154,6 @@ __global__ void __launch_bounds__(256, Waves<R>::electron) k_electron(KArgs<R> A
         if (!alive && !exhausted && ((uint32_t)__popcll(want) >= A.fetch_min || __ballot(alive) == 0)) {
             uint32_t i = wave_reserve(A.fetch, 1);
             if (i < in_end) {
-                if (A.perm) i = A.perm[i];
                 if (q_load(A.qin, i, p)) {
                     s.load(A.qin, i);
                     trips = A.qin.trips()[i];

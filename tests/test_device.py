@@ -271,13 +271,17 @@ def test_schedule_and_reruns_do_not_change_histories(oracle, D, tmp_path, case, 
         assert np.allclose(led, out[0][1], rtol=1e-12, atol=1e-12)
 
 
-@pytest.mark.parametrize("case", ["slab6MV", "positron6MeV"])
-def test_energy_ordered_fetch_keeps_histories(oracle, D, tmp_path, case):
+@pytest.mark.parametrize("case,ytrips", [("slab6MV", 0), ("positron6MeV", 0), ("slab6MV", 6)])
+def test_energy_ordered_fetch_keeps_histories(oracle, D, tmp_path, case, ytrips):
     """Charged-particle launches taking their input in order of falling
     energy (sort_min=1: every launch) run the same histories: each one's
-    ledger row, every event count and every voxel equal the oracle's."""
-    r = run_both(oracle, D, write_case(case, tmp_path, ncase=1000), sort_min=1)
+    ledger row, every event count and every voxel equal the oracle's.  With
+    a yield budget of 6 trips most charged particles yield and resume, so
+    the sort moves their resume state as well."""
+    r = run_both(oracle, D, write_case(case, tmp_path, ncase=1000), sort_min=1, yield_trips=ytrips)
     assert_same_histories(r, case + " sorted")
+    if ytrips:
+        assert r["st"].yields > 1000
 
 
 @pytest.mark.parametrize("case", ["electron6MeV", "slab6MV"])
