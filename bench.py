@@ -58,11 +58,12 @@ import numpy as np
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
-# The engine drives 6 pipeline streams plus a helper stream; with HIP's
-# default of 4 hardware queues per process two of them can share a queue and
-# serialise.  Give the process 8 before the HIP runtime starts.
-if int(os.environ.get("GPU_MAX_HW_QUEUES", "0") or 0) < 8:
-    os.environ["GPU_MAX_HW_QUEUES"] = "8"
+# The engine drives up to 12 pipeline streams (10 by default for FP32) plus
+# a helper stream; with HIP's default of 4 hardware queues per process
+# several of them share a queue and serialise.  Give the process 16 before
+# the HIP runtime starts.
+if int(os.environ.get("GPU_MAX_HW_QUEUES", "0") or 0) < 16:
+    os.environ["GPU_MAX_HW_QUEUES"] = "16"
 
 HBM_PEAK_GBS = 8000.0
 METRIC = "histories/s and p50/p99 beamlet-dose latency at 2% σ, FP32-batched vs FP64 GPU"

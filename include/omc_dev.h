@@ -51,8 +51,12 @@ typedef struct {
     int64_t hbm_budget;      /* bytes the context may hold; 0 = 90% of free HBM */
     int64_t chunk;           /* live particles per pipeline before new primaries
                                 are held back; 0 = automatic */
-    int     pipelines;       /* concurrent transport pipelines (streams);
-                                0 = automatic */
+    int     pipelines;       /* concurrent transport pipelines (streams),
+                                at most 16; 0 = automatic (10 for FP32, 6
+                                for FP64).  Each stream, and the context's
+                                helper stream, needs a hardware queue of its
+                                own: set GPU_MAX_HW_QUEUES >= pipelines + 1
+                                before HIP starts */
     int     yield_trips;     /* transport loop trips a particle may take in
                                 one launch before it continues in the next
                                 (its lane state travels with it, so results
@@ -244,7 +248,7 @@ int  omc_dev_beamlets_sparse(omc_dev_ctx *ctx, const omc_source_beamlets *src,
  * of `src` on `probe` histories (rp->nbatch batches; rp->nperbatch, ledger
  * and stats unused) under each setting of one option at a time, from the
  * context's current settings (coordinate descent, one pass):
- *   pipelines  2, 4, 6, 8 (at most GPU_MAX_HW_QUEUES - 1 when that is set)
+ *   pipelines  2, 4, 6, 8, 10, 12 (at most GPU_MAX_HW_QUEUES - 1 when that is set)
  *   chunk      1M, 2M, 4M, 8M live particles per pipeline (within the budget)
  *   yield_trips 1024, 4096, 16384;  drain_lanes 16, 32, 48;  drain_trips 8, 16, 32
  *   event_div  1, 2, 4;  fetch_min 1, 4, 8, 16

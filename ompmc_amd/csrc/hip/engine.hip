@@ -74,15 +74,18 @@ struct DevBuf {
 
 constexpr int MAX_JOBS = 4096;
 constexpr int MAX_SEGS = 512;
-constexpr int MAX_PIPES = 8;
-constexpr int DEFAULT_PIPES = 6;
+constexpr int MAX_PIPES = 16;
 // Defaults from sweeps of pipelines x chunk on the 64^3 slab phantom with
-// the 6 MV spectrum (20M histories, FP32 and FP64): throughput grows with the
-// particles per launch (lanes refill many times before the launch drains)
-// and with overlapping pipelines up to 6 (FP32 histories/s: 4 x 2M 100M,
-// 4 x 4M 106M, 6 x 4M 111-113M, 8 x 4M 108M; FP64 72M / 76M / 74-76M / 72M).
-// 6 pipeline streams and the helper stream fit the 8 hardware queues the
-// drivers ask for; 8 pipelines would share queues there.
+// the 6 MV spectrum (steps of 1e8 histories, 16 hardware queues): the FP32
+// engine gains from overlapping pipelines up to 10 (histories/s at 4M live
+// particles each: 6 pipelines 2.15e8, 7 2.18e8, 8 2.28e8, 10 2.38e8, 12
+// 2.38e8), while the FP64 engine, whose charged-particle kernel holds 2
+// waves per SIMD, is flat from 6 to 12 (1.07e8-1.09e8).  Each pipeline
+// stream and the helper stream need a hardware queue of their own
+// (GPU_MAX_HW_QUEUES=16, which the drivers set); pipelines that share a
+// queue serialise.
+constexpr int DEFAULT_PIPES_FP32 = 10;
+constexpr int DEFAULT_PIPES_FP64 = 6;
 constexpr int64_t DEFAULT_CHUNK = 1 << 22;
 constexpr uint32_t DEFAULT_YIELD_TRIPS = 4096;
 // Drain compaction: once a launch's input is used up, a wavefront with at
@@ -1448,7 +1451,10 @@ int omc_dev_create(omc_dev_ctx **out, const omc_dev_opts *opts)
     if (!rc) rc = dalloc(c, (void **)&c->d_jobs, MAX_JOBS * sizeof(DevJob));
     const size_t phb = o.precision == OMC_PREC_FP64 ? sizeof(DevPhantom<double>) : sizeof(DevPhantom<float>);
     if (!rc) rc = dalloc(c, &c->d_phantoms, MAX_PHANTOMS * phb);
-    if (!rc) rc = configure_pipes(c, o.pipelines > 0 ? o.pipelines : DEFAULT_PIPES, o.chunk > 0 ? o.chunk : DEFAULT_CHUNK);
+    if (!rc) rc = configure_pipes(c, o.pipelines > 0 ? o.pipelines
+                                                    : o.precision == OMC_PREC_FP32 ? DEFAULT_PIPES_FP32
+                                                                                   : DEFAULT_PIPES_FP64,
+                                       o.chunk > 0 ? o.chunk : DEFAULT_CHUNK);
     if (rc) { omc_dev_destroy(c); return rc; }
     *out = c;
     return OMC_OK;
@@ -1704,7 +1710,7 @@ static int tune_impl(omc_dev_ctx *c, const std::function<int()> &run, int64_t hi
         return OMC_OK;
     };
     const std::vector<int64_t> axes[8] = {
-        { 2, 4, 6, 8 },                              // pipelines
+        { 2, 4, 6, 8, 10, 12 },                      // pipelines
         { 1 << 20, 1 << 21, 1 << 22, 1 << 23 },      // live particles per pipeline
         { 1024, 4096, 16384 },                       // yield budget (trips)
         { 16, 32, 48 },                              // drain lanes

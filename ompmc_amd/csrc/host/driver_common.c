@@ -18,11 +18,12 @@ double drv_time(void)
 
 void drv_parse_args(drv_args *a, int argc, char **argv)
 {
-    /* the engine drives 6 pipeline streams plus a helper stream: give the
-     * process 8 hardware queues (HIP's default is 4) before HIP starts, so
-     * that no two pipelines share one */
+    /* the engine drives up to 10 pipeline streams by default (up to 12
+     * after tuning) plus a helper stream: give the process 16 hardware
+     * queues (HIP's default is 4) before HIP starts, so that no two
+     * pipelines share one */
     const char *q = getenv("GPU_MAX_HW_QUEUES");
-    if (!q || atoi(q) < 8) setenv("GPU_MAX_HW_QUEUES", "8", 1);
+    if (!q || atoi(q) < 16) setenv("GPU_MAX_HW_QUEUES", "16", 1);
     memset(a, 0, sizeof *a);
     const char *env = getenv("OMC_PRECISION");
     a->precision = (env && !strcmp(env, "fp32")) ? OMC_PREC_FP32 : OMC_PREC_FP64;

@@ -10,7 +10,7 @@ from cases import write_case
 pytestmark = pytest.mark.gpu
 
 CANDIDATES = {
-    "pipelines": {2, 4, 6, 8},
+    "pipelines": {2, 4, 6, 8, 10, 12},
     "yield_trips": {1024, 4096, 16384},
     "drain_lanes": {16, 32, 48},
     "drain_trips": {8, 16, 32},
