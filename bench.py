@@ -58,8 +58,8 @@ import numpy as np
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
-# The engine drives up to 12 pipeline streams (10 by default for FP32) plus
-# a helper stream; with HIP's default of 4 hardware queues per process
+# The engine drives 6 pipeline streams by default (up to 12 after tuning)
+# plus a helper stream; with HIP's default of 4 hardware queues per process
 # several of them share a queue and serialise.  Give the process 16 before
 # the HIP runtime starts.
 if int(os.environ.get("GPU_MAX_HW_QUEUES", "0") or 0) < 16:

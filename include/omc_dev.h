@@ -52,8 +52,8 @@ typedef struct {
     int64_t chunk;           /* live particles per pipeline before new primaries
                                 are held back; 0 = automatic */
     int     pipelines;       /* concurrent transport pipelines (streams),
-                                at most 16; 0 = automatic (10 for FP32, 6
-                                for FP64).  Each stream, and the context's
+                                at most 16; 0 = automatic (6; the autotuner
+                                tries up to 12).  Each stream, and the context's
                                 helper stream, needs a hardware queue of its
                                 own: set GPU_MAX_HW_QUEUES >= pipelines + 1
                                 before HIP starts */
@@ -97,8 +97,8 @@ typedef struct {
                                 summed in the order they end).  Batches in
                                 flight: as many as the limit leaves room
                                 for, but at least two and enough histories
-                                for half of pipelines x chunk; no split
-                                where that saves no grid.  Results equal the whole-tally ones to
+                                for half of pipelines x chunk, and at most
+                                64; no split where that saves no grid.  Results equal the whole-tally ones to
                                 FP64 summation order.  0 = 512 MiB, < 0 =
                                 never split */
     int64_t sort_min;        /* charged-particle launches whose input holds at

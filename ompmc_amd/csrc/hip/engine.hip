@@ -75,18 +75,24 @@ struct DevBuf {
 constexpr int MAX_JOBS = 4096;
 constexpr int MAX_SEGS = 512;
 constexpr int MAX_PIPES = 16;
-// Defaults from sweeps of pipelines x chunk on the 64^3 slab phantom with
-// the 6 MV spectrum (steps of 1e8 histories, 16 hardware queues): the FP32
-// engine gains from overlapping pipelines up to 10 (histories/s at 4M live
-// particles each: 6 pipelines 2.15e8, 7 2.18e8, 8 2.28e8, 10 2.38e8, 12
-// 2.38e8), while the FP64 engine, whose charged-particle kernel holds 2
-// waves per SIMD, is flat from 6 to 12 (1.07e8-1.09e8).  Each pipeline
-// stream and the helper stream need a hardware queue of their own
-// (GPU_MAX_HW_QUEUES=16, which the drivers set); pipelines that share a
-// queue serialise.
-constexpr int DEFAULT_PIPES_FP32 = 10;
+// Pipelines x chunk on the 64^3 slab phantom with the 6 MV spectrum (steps
+// of 1e8 histories, 16 hardware queues): the FP32 engine gains from
+// overlapping pipelines up to 10 (histories/s at 4M live particles each: 6
+// pipelines 2.15e8, 7 2.18e8, 8 2.28e8, 10 2.38e8, 12 2.38e8), while the
+// FP64 engine, whose charged-particle kernel holds 2 waves per SIMD, is flat
+// from 6 to 12 (1.07e8-1.09e8).  The default stays at 6 for both: the
+// serving loop has been exercised at 6, and the autotuner reaches 10-12
+// where a workload gains.  Each pipeline stream and the helper stream need
+// a hardware queue of their own (GPU_MAX_HW_QUEUES=16, which the drivers
+// set); pipelines that share a queue serialise.
+constexpr int DEFAULT_PIPES_FP32 = 6;
 constexpr int DEFAULT_PIPES_FP64 = 6;
 constexpr int64_t DEFAULT_CHUNK = 1 << 22;
+// Batches of one work admitted batch by batch that may be in flight at once
+// (each is a job with its own grid).  Runs have held up to 13; a serving
+// request of about 2000 small batches in flight faulted the device with a
+// cause not yet found, so the count is held to a range that has run.
+constexpr int64_t MAX_SPLIT_INFLIGHT = 64;
 constexpr uint32_t DEFAULT_YIELD_TRIPS = 4096;
 // Drain compaction: once a launch's input is used up, a wavefront with at
 // most 32 live lanes hands its particles (after 16 trips in the launch) to
@@ -1314,8 +1320,9 @@ static void plan_split(const omc_dev_ctx *c, Work &w)
     if (c->tally_cap <= 0 || w.scratch || w.nbatch < 2 || w.slots != w.nbatch) return;
     if (grid * w.nbatch <= c->tally_cap) return;
     const int64_t inflight = (int64_t)c->pipes.size() * c->chunk / 2;
-    const int64_t k = std::max<int64_t>({ c->tally_cap / grid - 2, (inflight + w.nperbatch - 1) / w.nperbatch,
-                                         (int64_t)2 });
+    const int64_t k = std::min<int64_t>(std::max<int64_t>({ c->tally_cap / grid - 2,
+                                                            (inflight + w.nperbatch - 1) / w.nperbatch, (int64_t)2 }),
+                                        MAX_SPLIT_INFLIGHT);
     if (k + 2 >= w.nbatch) return;
     w.split_k = (int)k;
 }

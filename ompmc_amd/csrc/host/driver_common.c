@@ -18,10 +18,10 @@ double drv_time(void)
 
 void drv_parse_args(drv_args *a, int argc, char **argv)
 {
-    /* the engine drives up to 10 pipeline streams by default (up to 12
-     * after tuning) plus a helper stream: give the process 16 hardware
-     * queues (HIP's default is 4) before HIP starts, so that no two
-     * pipelines share one */
+    /* the engine drives 6 pipeline streams by default (up to 12 after
+     * tuning) plus a helper stream: give the process 16 hardware queues
+     * (HIP's default is 4) before HIP starts, so that no two pipelines
+     * share one */
     const char *q = getenv("GPU_MAX_HW_QUEUES");
     if (!q || atoi(q) < 16) setenv("GPU_MAX_HW_QUEUES", "16", 1);
     memset(a, 0, sizeof *a);
