@@ -85,8 +85,7 @@ constexpr int MAX_PIPES = 16;
 // where a workload gains.  Each pipeline stream and the helper stream need
 // a hardware queue of their own (GPU_MAX_HW_QUEUES=16, which the drivers
 // set); pipelines that share a queue serialise.
-constexpr int DEFAULT_PIPES_FP32 = 6;
-constexpr int DEFAULT_PIPES_FP64 = 6;
+constexpr int DEFAULT_PIPES = 6;
 constexpr int64_t DEFAULT_CHUNK = 1 << 22;
 // Batches of one work admitted batch by batch that may be in flight at once
 // (each is a job with its own grid).  Runs have held up to 13; a serving
@@ -1458,10 +1457,7 @@ int omc_dev_create(omc_dev_ctx **out, const omc_dev_opts *opts)
     if (!rc) rc = dalloc(c, (void **)&c->d_jobs, MAX_JOBS * sizeof(DevJob));
     const size_t phb = o.precision == OMC_PREC_FP64 ? sizeof(DevPhantom<double>) : sizeof(DevPhantom<float>);
     if (!rc) rc = dalloc(c, &c->d_phantoms, MAX_PHANTOMS * phb);
-    if (!rc) rc = configure_pipes(c, o.pipelines > 0 ? o.pipelines
-                                                    : o.precision == OMC_PREC_FP32 ? DEFAULT_PIPES_FP32
-                                                                                   : DEFAULT_PIPES_FP64,
-                                       o.chunk > 0 ? o.chunk : DEFAULT_CHUNK);
+    if (!rc) rc = configure_pipes(c, o.pipelines > 0 ? o.pipelines : DEFAULT_PIPES, o.chunk > 0 ? o.chunk : DEFAULT_CHUNK);
     if (rc) { omc_dev_destroy(c); return rc; }
     *out = c;
     return OMC_OK;
