@@ -168,12 +168,13 @@ void Launch<R>::math_eval(int fn, int64_t n, const double *x, const double *y, d
 }
 
 // ---- energy-ordered fetch of a charged-particle launch ---------------------
-// A counting sort of the queue's indices into SORT_BUCKETS buckets of total
+// A counting sort of the queue's records into SORT_BUCKETS buckets of total
 // energy, highest first, so that the lanes of a wavefront start on particles
 // of similar energy (their table lookups fall on few cache lines, their
-// steps are alike) and the longest tracks start first.  Stable within a
-// block's share; the histories do not depend on the order.  The queue's
-// final count is read on the device: the photon kernel before it appends.
+// steps are alike) and the longest tracks start first.  The order within a
+// bucket follows the LDS atomics and is not fixed; the histories do not
+// depend on it.  The queue's count is read on the device, after the photon
+// kernel on the same stream has appended to the queue.
 
 // Bucket of total energy e: 12 per octave from 0.5 MeV (below every
 // charged-particle cut-off) up, highest energy in bucket 0.
