@@ -116,15 +116,14 @@ constexpr int64_t drain_min_default(int cus, int precision)
 // of 16 / 4 / 8 / 1 lanes (FP64 and the beamlet batch within noise).
 constexpr uint32_t DEFAULT_EVENT_DIV = 2;
 // Whole batch tallies up to 512 MiB per job (the C3 beamlet's 10 grids of the
-// 128x128x96 phantom take 126 MB); a C5-sized job (84 MB per grid) is
-// admitted batch by batch, four batches in flight beside its sum grids.
+// 128x128x96 phantom take 126 MB); a C5-sized job (84 MB per grid, 100
+// batches) is admitted batch by batch, its batches in flight beside its sum
+// grids.
 constexpr int64_t DEFAULT_TALLY_CAP = (int64_t)512 << 20;
-// Energy-ordered fetch of charged-particle launches whose input exceeds the
-// lanes the device holds resident (drain_min_default): only then does the
-// order in which particles start matter.  Slab, 5e7 histories: FP32 1.80e8
-// -> 1.97e8 histories/s (charged-particle kernel time -15%), FP64 9.5e7 ->
-// 1.03e8; the histories are the same.
-constexpr uint32_t DEFAULT_FETCH_MIN = 1;
+// Free lanes of a wavefront take new particles once this many are free (or
+// none is live).  Slab, steps of 1e8 histories, FP32 / FP64 histories/s:
+// 1 2.20e8 / 1.07e8, 2 2.23e8 / 1.08e8, 4 2.29e8 / 1.10e8, 8 2.26e8 / 1.10e8.
+constexpr uint32_t DEFAULT_FETCH_MIN = 4;
 
 struct PhantomSlot {
     bool used = false;
@@ -1447,6 +1446,9 @@ int omc_dev_create(omc_dev_ctx **out, const omc_dev_opts *opts)
     c->drain_min = o.drain_min > 0 ? o.drain_min : o.drain_min < 0 ? 0 : drain_min_default(c->cus, c->precision);
     c->woodcock = o.woodcock > 0 || (o.woodcock == 0 && o.precision == OMC_PREC_FP32);
     c->tally_cap = o.tally_bytes > 0 ? o.tally_bytes : o.tally_bytes < 0 ? 0 : DEFAULT_TALLY_CAP;
+    // energy-ordered charged-particle launches once their input exceeds the
+    // lanes the device holds resident: only then does the order in which
+    // particles start matter (slab: FP32 +10%, FP64 +8%; same histories)
     c->sort_min = o.sort_min > 0 ? o.sort_min : o.sort_min < 0 ? 0 : drain_min_default(c->cus, c->precision);
     size_t freeb = 0, totalb = 0;
     (void)hipMemGetInfo(&freeb, &totalb);

@@ -1068,8 +1068,11 @@ __global__ void __launch_bounds__(256, Waves<R>::photon) k_photon(KArgs<R> A)
         // free lanes take new particles together (one counter atomic and one
         // round of queue loads for all of them), once enough are free or
         // none is live
+        // (both ballots are taken by the whole wavefront: inside the branch
+        // only free lanes are active, and a ballot of `alive` there is 0)
         const unsigned long long want = __ballot(!alive && !exhausted);
-        if (!alive && !exhausted && ((uint32_t)__popcll(want) >= A.fetch_min || __ballot(alive) == 0)) {
+        const bool none_live = __ballot(alive) == 0;
+        if (!alive && !exhausted && ((uint32_t)__popcll(want) >= A.fetch_min || none_live)) {
             uint32_t i = wave_reserve(A.fetch, 1);
             if (i < in_end) {
                 if (q_load(A.qin, i, p)) {
@@ -1150,8 +1153,11 @@ __global__ void __launch_bounds__(256, Waves<R>::electron) k_electron(KArgs<R> A
         // free lanes take new particles together (one counter atomic and one
         // round of queue loads for all of them), once enough are free or
         // none is live
+        // (both ballots are taken by the whole wavefront: inside the branch
+        // only free lanes are active, and a ballot of `alive` there is 0)
         const unsigned long long want = __ballot(!alive && !exhausted);
-        if (!alive && !exhausted && ((uint32_t)__popcll(want) >= A.fetch_min || __ballot(alive) == 0)) {
+        const bool none_live = __ballot(alive) == 0;
+        if (!alive && !exhausted && ((uint32_t)__popcll(want) >= A.fetch_min || none_live)) {
             uint32_t i = wave_reserve(A.fetch, 1);
             if (i < in_end) {
                 if (q_load(A.qin, i, p)) {

@@ -242,20 +242,22 @@ def test_schedule_and_reruns_do_not_change_histories(oracle, D, tmp_path, case, 
     older secondaries), particles handed to the next launch after every
     one or three loop trips (their lane state travels with them), handed
     on after every trip once a launch's input is used up, not drained at
-    all, or a repeat reproduce the same energies up to FP64 summation
-    order."""
+    all, free lanes waiting until 16 of a wavefront are free before they
+    take new particles, or a repeat reproduce the same energies up to FP64
+    summation order."""
     stem = write_case(case, tmp_path, ncase=3000)
     prob = oracle.load_problem(stem)
     try:
         out = []
-        for chunk, pipes, ytrips, drain in ((0, 0, 0, -1), (0, 1, 0, 0), (777, 3, 0, 0), (0, 2, 1, 0),
-                                            (5000, 0, 3, 0), (2000, 2, 0, 64), (0, 0, 0, None)):
+        for chunk, pipes, ytrips, drain, fmin in ((0, 0, 0, -1, 0), (0, 1, 0, 0, 0), (777, 3, 0, 0, 0),
+                                                  (0, 2, 1, 0, 0), (5000, 0, 3, 0, 0), (2000, 2, 0, 64, 0),
+                                                  (0, 0, 0, None, 0), (777, 2, 3, 0, 16)):
             # drain_min -1: these small launches drain too (by default, the
-            # last configuration, a launch smaller than the device's
+            # seventh configuration, a launch smaller than the device's
             # resident lanes does not)
             dm = 0 if drain is None or drain < 0 else -1
             with D.Engine(precision, chunk=chunk, pipelines=pipes, yield_trips=ytrips, drain_lanes=drain or 0,
-                          drain_trips=1 if drain == 64 else 0, drain_min=dm) as eng:
+                          drain_trips=1 if drain == 64 else 0, drain_min=dm, fetch_min=fmin) as eng:
                 eng.upload_media(prob.tab)
                 pid = eng.upload_phantom(prob.ph, prob.reg)
                 s, _, st, led = eng.run_dosxyz(pid, prob.src, prob.nperbatch, prob.nbatch, seed=(4, 4),
