@@ -71,8 +71,8 @@ typedef struct {
                                 interactions once 1/event_div of its live
                                 lanes wait for one; 0 = automatic */
     int     fetch_min;       /* free lanes of a wavefront take new particles
-                                together once this many are free;
-                                0 = automatic */
+                                together once this many are free (or none
+                                is live); 0 = automatic (4) */
     int64_t drain_min;       /* launches whose input holds fewer particles
                                 than this do not drain: on a device they
                                 cannot fill, repacking stragglers gains
