@@ -7,6 +7,24 @@
 
 namespace omc {
 
+// The voxel index the reference's linear scans find, `i = lo; while (i < hi
+// && b[i + 1] < x) ++i` (omc_dosxyz.c:1040-1055, omc_matrad.c:1236-1248),
+// found by bisection: the bounds ascend, so `b[i + 1] < x` holds for a
+// prefix of [lo, hi) and the scan stops at the first i where it fails, or
+// at hi.  The comparisons are the scan's own (bound rounded to R first), so
+// the index is the same; a thread reads log2(hi - lo) bounds instead of up
+// to hi - lo, one after the other.
+template <typename R>
+__device__ __forceinline__ int scan_bounds(const double *b, int lo, int hi, R x)
+{
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if ((R)b[mid + 1] < x) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
 // initHistory for a dosxyz field (omc_dosxyz.c:964-1068) or a matRad beamlet
 // (omc_matrad.c:1084-1254): one thread per history, primary pushed to the
 // photon or charged-particle queue.  One launch serves `nseg` segments of
@@ -71,18 +89,8 @@ __global__ void __launch_bounds__(256) k_source(const DevJob *jobs, const DevPha
         p.u = p.x / rxyz;
         p.v = p.y / rxyz;
         int ix, iy;
-        if (J.xsize == 0.0) {
-            ix = J.ixinl;
-        } else {
-            ix = J.ixinl - 1;
-            while (((R)g.xb[ix + 1] < p.x) && ix < g.isize - 1) ix++;
-        }
-        if (J.ysize == 0.0) {
-            iy = J.iyinl;
-        } else {
-            iy = J.iyinl - 1;
-            while (((R)g.yb[iy + 1] < p.y) && iy < g.jsize - 1) iy++;
-        }
+        ix = J.xsize == 0.0 ? J.ixinl : scan_bounds<R>(g.xb, J.ixinl - 1, g.isize - 1, p.x);
+        iy = J.ysize == 0.0 ? J.iyinl : scan_bounds<R>(g.yb, J.iyinl - 1, g.jsize - 1, p.y);
         p.ir = 1 + ix + iy * g.isize;
     } else {
         R r1 = urand<R>(p.rng), r2 = urand<R>(p.rng);
@@ -110,10 +118,9 @@ __global__ void __launch_bounds__(256) k_source(const DevJob *jobs, const DevPha
         if (p.y > (R)g.yb[g.jsize]) p.y = (R)g.yb[g.jsize] - tiny;
         if (p.z < (R)g.zb[0]) p.z = (R)g.yb[0] + tiny;      // reference's ybounds (omc_matrad.c:1225)
         if (p.z > (R)g.zb[g.ksize]) p.z = (R)g.zb[g.ksize] - tiny;
-        int ix = 0, iy = 0, iz = 0;
-        while (ix < g.isize - 1 && (R)g.xb[ix + 1] < p.x) ix++;
-        while (iy < g.jsize - 1 && (R)g.yb[iy + 1] < p.y) iy++;
-        while (iz < g.ksize - 1 && (R)g.zb[iz + 1] < p.z) iz++;
+        const int ix = scan_bounds<R>(g.xb, 0, g.isize - 1, p.x);
+        const int iy = scan_bounds<R>(g.yb, 0, g.jsize - 1, p.y);
+        const int iz = scan_bounds<R>(g.zb, 0, g.ksize - 1, p.z);
         p.ir = 1 + ix + iy * g.isize + iz * g.isize * g.jsize;
     }
     push<R>(p.iq == 0 ? qp : qe, p, st);
