@@ -98,7 +98,7 @@ typedef struct {
                                 flight: as many as the limit leaves room
                                 for, but at least two and enough histories
                                 for half of pipelines x chunk, and at most
-                                64; no split where that saves no grid.  Results equal the whole-tally ones to
+                                16; no split where that saves no grid.  Results equal the whole-tally ones to
                                 FP64 summation order.  0 = 512 MiB, < 0 =
                                 never split */
     int64_t sort_min;        /* charged-particle launches whose input holds at

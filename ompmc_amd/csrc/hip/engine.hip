@@ -90,8 +90,8 @@ constexpr int64_t DEFAULT_CHUNK = 1 << 22;
 // Batches of one work admitted batch by batch that may be in flight at once
 // (each is a job with its own grid).  Runs have held up to 13; a serving
 // request of about 2000 small batches in flight faulted the device with a
-// cause not yet found, so the count is held to a range that has run.
-constexpr int64_t MAX_SPLIT_INFLIGHT = 64;
+// cause not yet found, so the count is held near the range that has run.
+constexpr int64_t MAX_SPLIT_INFLIGHT = 16;
 constexpr uint32_t DEFAULT_YIELD_TRIPS = 4096;
 // Drain compaction: once a launch's input is used up, a wavefront with at
 // most 32 live lanes hands its particles (after 16 trips in the launch) to
